@@ -1,0 +1,240 @@
+"""Benchmark of the convective + diffusive residual pair on one MI355X.
+
+Workload (default): unit cube of 128x128x128 hexahedra of order 4
+(135,005,697 global GLL nodes, 262,144,000 element nodes), FP64, synthetic
+fields from the host generator.  A step is one call of the convective and one
+of the diffusive residual over the whole mesh (zero-fill + kernel each, the
+reference's timed region, src/app/sources/main.f90:446-478), with every input
+already resident in HBM.
+
+Prints one JSON line: GLL-node updates/s = npoin / (t_convec + t_diffu), the
+roofline of the dominant kernel from HIP-event durations and the canonical
+algorithmic bytes, the mixed-precision variant beside it, a sampled-node
+parity check against the CPU oracle, and the reference's own CPU kernels timed
+on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import minisod2d_amd as m  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X, bytes/s
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def canonical_bytes(E, N, wg, wf):
+    """Algorithmic bytes of one convective and one diffusive call: each input
+    element read once, each output written once (FP64).  wg/wf: bytes per
+    geometry/field word."""
+    bc = E * (4 + 10 * wg) + N * (9 * wf + 40)
+    bd = E * (4 + 10 * wg + 2 * wf) + N * (6 * wf + 40)
+    return bc, bd
+
+
+def run_pair(prob, R, D, mixed, f32=None):
+    t = prob.tables
+    if mixed:
+        m.full_convec_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
+                                t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, f32["u"],
+                                f32["q"], f32["rho"], f32["E"], prob.gamma, *R)
+        m.full_diffusion_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
+                                   t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp,
+                                   prob.Pr, f32["rho"], f32["u"], f32["Tem"], f32["mu_fluid"],
+                                   f32["mu_e"], f32["mu_sgs"], *D)
+    else:
+        m.full_convec_ijk(*prob.convec_args(*R))
+        m.full_diffusion_ijk(*prob.diffu_args(*D))
+
+
+def timed(prob, R, D, steps, warmup, mixed, f32=None):
+    m.set_async(True)
+    for _ in range(warmup):
+        run_pair(prob, R, D, mixed, f32)
+    m.synchronize()
+    m.timing_reset()
+    m.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_pair(prob, R, D, mixed, f32)
+    m.synchronize()
+    t1 = time.perf_counter()
+    m.timing_enable(False)
+    m.set_async(False)
+    kc, kd = (m.TIMER_CONVEC_MIXED, m.TIMER_DIFFU_MIXED) if mixed else (m.TIMER_CONVEC, m.TIMER_DIFFU)
+    zc = m.timing_get(m.TIMER_ZERO)
+    c = m.timing_get(kc)
+    d = m.timing_get(kd)
+    return dict(ms_per_step=(t1 - t0) * 1e3 / steps,
+                convec_ms=c[0] / max(c[1], 1), diffu_ms=d[0] / max(d[1], 1),
+                zero_ms=zc[0] / max(zc[1], 1))
+
+
+def sampled_parity(prob, out, nsample=3000, seed=7):
+    """Residuals at randomly chosen nodes against the oracle run on exactly
+    the elements that touch those nodes (each sampled node then receives all
+    of its contributions)."""
+    from oracle_lib import Oracle, rel_l2
+    rng = np.random.default_rng(seed)
+    nodes = rng.choice(prob.npoin, size=min(nsample, prob.npoin), replace=False) + 1
+    hit = np.isin(prob.connec, nodes).any(axis=1)
+    els = np.nonzero(hit)[0]
+    sub_conn = prob.connec[els]
+    uniq, inv = np.unique(sub_conn, return_inverse=True)
+    sub = m.Problem(prob.tables, len(els), len(uniq),
+                    np.asfortranarray((inv.reshape(sub_conn.shape) + 1).astype(np.int32)),
+                    prob.coord[uniq - 1], np.asfortranarray(prob.He[:, :, :, els]),
+                    np.asfortranarray(prob.gpvol[:, :, els]),
+                    *[np.asfortranarray(getattr(prob, k)[uniq - 1]) for k in
+                      ("u", "q", "rho", "pr", "E", "Tem", "mu_fluid")],
+                    np.asfortranarray(prob.mu_e[els]), np.asfortranarray(prob.mu_sgs[els]),
+                    Cp=prob.Cp, Pr=prob.Pr, gamma=prob.gamma)
+    o = Oracle()
+    want = o.convec(sub, 8) + o.diffu(sub, 8)
+    pos = np.searchsorted(uniq, nodes)
+    errs = {}
+    for key, g, w in zip(("Rmass", "Rmom", "Rener", "Dmass", "Dmom", "Dener"), out, want):
+        errs[key] = rel_l2(g[nodes - 1], w[pos])
+    return errs
+
+
+def cpu_baseline(porder, n=32, nruns=3):
+    """The reference's own Fortran kernels (oracle/_ref, compiled from the
+    reference sources, rp=8, single thread as the reference is) on an
+    n^3-element box of the same generator."""
+    from oracle_lib import RefLib, ref_available
+    if not ref_available(porder):
+        return None
+    ref = RefLib(porder)
+    prob = m.box_problem(n, porder, jitter=0.0)
+    t = prob.tables
+    R = prob.outputs()
+    D = prob.outputs()
+    p_ = lambda a: a.ctypes.data
+    cargs = (prob.nelem, prob.npoin, p_(prob.connec), p_(t.Ngp), p_(t.dNgp), p_(prob.He),
+             p_(prob.gpvol), p_(t.dlxigp_ip), p_(t.xgp), p_(t.invAtoIJK), p_(t.AtoI), p_(t.AtoJ),
+             p_(t.AtoK), p_(prob.u), p_(prob.q), p_(prob.rho), p_(prob.pr), p_(prob.E),
+             *map(p_, R))
+    tc, td = [], []
+    for _ in range(nruns):
+        t0 = time.perf_counter()
+        ref.L.sodref_full_convec_ijk(*cargs)
+        t1 = time.perf_counter()
+        ref.diffu(prob)
+        t2 = time.perf_counter()
+        tc.append(t1 - t0)
+        td.append(t2 - t1)
+    tcm, tdm = float(np.median(tc)), float(np.median(td))
+    return dict(value=prob.npoin / (tcm + tdm), unit="GLL-node updates/s", cores=1,
+                kind="reference",
+                sample=f"{n}^3 elements porder {porder} ({prob.npoin} nodes), reference "
+                       f"full_convec_ijk + full_diffusion_ijk (amdflang -O3, rp=8), median of "
+                       f"{nruns}: convec {tcm*1e3:.1f} ms, diffu {tdm*1e3:.1f} ms")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=128, help="elements per axis")
+    ap.add_argument("--porder", type=int, default=4)
+    ap.add_argument("--jitter", type=float, default=0.0)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported here "
+              f"(the element-partitioned multi-GPU form is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    n, p = args.n, args.porder
+    log(f"generating {n}^3 porder {p} (host, {args.threads} threads)")
+    t0 = time.time()
+    prob = m.box_problem(n, p, jitter=args.jitter, threads=min(args.threads, os.cpu_count() or 1))
+    N, E = prob.npoin, prob.nelem * prob.tables.nnode
+    log(f"generated npoin={N} elem-nodes={E} in {time.time()-t0:.1f}s")
+
+    m.init(0)
+    R, D = prob.outputs(), prob.outputs()
+    ins = prob.device_inputs()
+    t0 = time.time()
+    m.copyin(*ins)
+    m.create(*R, *D)
+    log(f"uploaded in {time.time()-t0:.1f}s; timing FP64")
+    r64 = timed(prob, R, D, args.steps, args.warmup, mixed=False)
+    log(f"FP64 {r64}")
+    parity = None
+    if not args.no_parity:
+        m.update_host(*R, *D)
+        parity = sampled_parity(prob, R + D)
+        log(f"sampled parity {parity}")
+
+    rmx = None
+    if not args.no_mixed:
+        f32 = prob.fp32()
+        m.copyin(*[f32[k] for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")])
+        log("timing mixed precision")
+        rmx = timed(prob, R, D, args.steps, args.warmup, mixed=True, f32=f32)
+        log(f"mixed {rmx}")
+        if parity is not None:
+            m.update_host(*R, *D)
+            rmx["sampled_rel_l2"] = sampled_parity(prob, R + D)
+
+    bc, bd = canonical_bytes(E, N, 8, 8)
+    dom, dom_ms, dom_b = ("full_convec_ijk", r64["convec_ms"], bc)
+    if r64["diffu_ms"] > r64["convec_ms"]:
+        dom, dom_ms, dom_b = ("full_diffusion_ijk", r64["diffu_ms"], bd)
+    achieved = dom_b / (dom_ms * 1e-3)
+    value = N / (r64["ms_per_step"] * 1e-3)
+    rec = {
+        "metric": "GLL-node updates/s (convec+diffu), FP64 vs FP32-split at stated err; HBM GB/s",
+        "value": value,
+        "unit": "GLL-node updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"unit cube {n}x{n}x{n} hex elements, porder {p}, "
+                               f"{N} GLL nodes, {E} element nodes, jitter {args.jitter}",
+                   "npoin": N, "nelem": prob.nelem, "porder": p},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": None},
+        "kernels_ms": {"convec": r64["convec_ms"], "diffu": r64["diffu_ms"],
+                       "zero_fill": r64["zero_ms"]},
+        "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,
+        "parity_sampled_rel_l2": parity,
+    }
+    if rmx is not None:
+        bcm, bdm = canonical_bytes(E, N, 4, 4)
+        rec["mixed"] = {"value": N / (rmx["ms_per_step"] * 1e-3), "ms_per_step": rmx["ms_per_step"],
+                        "speedup_vs_fp64": r64["ms_per_step"] / rmx["ms_per_step"],
+                        "kernels_ms": {"convec": rmx["convec_ms"], "diffu": rmx["diffu_ms"]},
+                        "hbm_frac_convec": bcm / (rmx["convec_ms"] * 1e-3) / HBM_PEAK,
+                        "hbm_frac_diffu": bdm / (rmx["diffu_ms"] * 1e-3) / HBM_PEAK,
+                        "rel_l2_vs_fp64_oracle": rmx.get("sampled_rel_l2")}
+    m.finalize()
+    del prob, R, D, ins
+    if not args.no_cpu_baseline:
+        log("timing the reference CPU kernels on a bounded sample")
+        rec["cpu_baseline"] = cpu_baseline(p)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * msod2d.h -- C ABI of the MI355X (gfx950) spectral-element residual kernels.
+ *
+ * Drop-in boundary for the two timed kernels of miniSOD2D:
+ *   full_convec_ijk     src/kernels/sources/elem_convec.f90:198-211
+ *   full_diffusion_ijk  src/kernels/sources/elem_diffu.f90:10-23
+ * The reference passes every array by reference from Fortran with OpenACC
+ * device copies made once by `!$acc enter data` in the driver
+ * (src/app/sources/main.f90:197-425); the kernels then run on those copies
+ * (`present`).  This ABI keeps that model:
+ *
+ *   1. msod_data_* mirror host arrays on the GPU, keyed by host address
+ *      (the `enter data` / `update` / `exit data` counterparts);
+ *   2. msod_full_convec_ijk / msod_full_diffusion_ijk take the reference's
+ *      argument list (porder first, since it is a compile-time constant in
+ *      the reference, src/utils/sources/mod_constants.f90:16) and resolve
+ *      every array pointer to its device mirror.  A pointer that is itself
+ *      device memory (hipMalloc) is used as is (OpenACC `deviceptr`).
+ *
+ * Arrays keep the reference's Fortran layouts and 1-based index values:
+ *   connec(nelem,nnode) int32; He(3,3,ngaus,nelem); gpvol(1,ngaus,nelem);
+ *   dlxigp_ip(ngaus,3,porder+1); invAtoIJK(q,q,q), AtoI/J/K(nnode) int32;
+ *   u,q(npoin,3); rho,pr,E,Tem,mu_fluid(npoin); mu_e,mu_sgs(nelem,ngaus);
+ *   Rmass(npoin), Rmom(npoin,3), Rener(npoin)  (intent(out): zeroed inside)
+ * with q = porder+1 and nnode = ngaus = q^3.  Ngp, dNgp and xgp are accepted
+ * and ignored: the reference kernels never read them.
+ *
+ * Every function returns a status (0 = MSOD_OK); msod_last_error() gives the
+ * message of the last failure.  Calls are synchronous by default (like the
+ * reference's compute regions); msod_set_async(1) lets kernel calls return
+ * once queued on the library's stream.
+ */
+#ifndef MSOD2D_H
+#define MSOD2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum msod_status {
+    MSOD_OK = 0,
+    MSOD_ERR_ARG = 1,         /* invalid scalar argument (sizes, porder range, NULL) */
+    MSOD_ERR_NOT_PRESENT = 2, /* array neither in the data region nor device memory */
+    MSOD_ERR_PORDER = 3,      /* porder outside the compiled range 1..8 */
+    MSOD_ERR_TABLE = 4,       /* invAtoIJK / AtoI,J,K / dlxigp_ip inconsistent */
+    MSOD_ERR_HIP = 5,         /* HIP runtime failure */
+    MSOD_ERR_STATE = 6,       /* library not initialised */
+    MSOD_ERR_MESH = 7,        /* connec entry outside 1..npoin or array too small */
+    MSOD_ERR_MODEL = 8        /* learned-closure weights missing or not finite */
+};
+
+/* ---- library state ---------------------------------------------------- */
+int msod_init(int device);
+int msod_finalize(void);
+const char *msod_status_string(int status);
+const char *msod_last_error(void);
+
+/* ---- data region: replaces `!$acc enter data / update / exit data` of
+ *      src/app/sources/main.f90:197-425,539-542 --------------------------- */
+int msod_data_copyin(const void *host, size_t bytes);
+int msod_data_create(const void *host, size_t bytes);
+int msod_data_update_device(const void *host, size_t bytes);
+int msod_data_update_host(void *host, size_t bytes);
+int msod_data_delete(const void *host);
+int msod_data_present(const void *host);         /* 1 if present, else 0 */
+void *msod_data_deviceptr(const void *host);     /* NULL if not present */
+/* Forget layouts derived from a device array changed outside msod_data_* */
+int msod_data_invalidate(const void *ptr);
+
+/* ---- execution control -------------------------------------------------- */
+int msod_set_async(int async);
+int msod_synchronize(void);
+/* HIP-event timing of every kernel launch on the library stream */
+int msod_timing_enable(int enable);
+/* kernel: 0 zero-fill, 1 convec, 2 diffu, 3 convec mixed, 4 diffu mixed,
+ *         5 diffu learned closure.  Returns accumulated ms and launch count. */
+int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
+int msod_timing_reset(void);
+
+/* ---- residual kernels --------------------------------------------------- */
+
+/* Convective residual, FP64.  Rmass/Rmom as full_convec_ijk
+ * (elem_convec.f90:198-431); Rener with the kinetic-energy flux terms
+ * fuel = E*u, fukl = k*u of full_convec_ijk_split (elem_convec.f90:694-697),
+ * which the fused reference kernel leaves uninitialised (:218,:248-250). */
+int msod_full_convec_ijk(int porder, int nelem, int npoin, const int32_t *connec,
+                         const double *Ngp, const double *dNgp, const double *He,
+                         const double *gpvol, const double *dlxigp_ip, const double *xgp,
+                         const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                         const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const double *u,
+                         const double *q, const double *rho, const double *pr,
+                         const double *E, double *Rmass, double *Rmom, double *Rener);
+
+/* Diffusive residual, FP64 (elem_diffu.f90:10-197).  c_rho and c_ener are
+ * module globals of the reference (mod_numerical_params.f90:26-27); set them
+ * with msod_set_diffusion_coefficients (default 1, 1). */
+int msod_full_diffusion_ijk(int porder, int nelem, int npoin, const int32_t *connec,
+                            const double *Ngp, const double *dNgp, const double *He,
+                            const double *gpvol, const double *dlxigp_ip, const double *xgp,
+                            const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                            const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                            double Pr, const double *rho, const double *u, const double *Tem,
+                            const double *mu_fluid, const double *mu_e, const double *mu_sgs,
+                            double *Rmass, double *Rmom, double *Rener);
+int msod_set_diffusion_coefficients(double c_rho, double c_ener);
+
+/* Mixed-precision forms (no counterpart in the reference, whose single real
+ * kind rp is global, mod_constants.f90:5): geometry, fields, fluxes and
+ * derivative sweeps in FP32, nodal assembly in FP64 atomics.  The convective
+ * form takes no pressure array: it recovers p = (gamma-1)(E - rho|u|^2/2) in
+ * FP64 from the FP32 conservative state. */
+int msod_full_convec_ijk_mixed(int porder, int nelem, int npoin, const int32_t *connec,
+                               const float *He, const float *gpvol, const double *dlxigp_ip,
+                               const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                               const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                               const float *u, const float *q, const float *rho, const float *E,
+                               double gamma_gas, double *Rmass, double *Rmom, double *Rener);
+int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_t *connec,
+                                  const float *He, const float *gpvol, const double *dlxigp_ip,
+                                  const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                                  const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                                  double Pr, const float *rho, const float *u, const float *Tem,
+                                  const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                  double *Rmass, double *Rmom, double *Rener);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSOD2D_H */

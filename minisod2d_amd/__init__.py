@@ -1,0 +1,424 @@
+"""MI355X-native spectral-element residual kernels of miniSOD2D.
+
+Python host mirror of the reference's kernel interface: the functions below
+take the same arguments, in the same order and Fortran layouts, as
+``full_convec_ijk`` (src/kernels/sources/elem_convec.f90:198) and
+``full_diffusion_ijk`` (src/kernels/sources/elem_diffu.f90:10), and run the
+hand-written HIP kernels of ``libmsod2d_hip.so`` through its C ABI
+(include/msod2d.h).  There is no CPU fallback: without the built library or
+a GPU every call raises.
+
+Arrays are numpy arrays in Fortran order.  Like the reference, which expects
+every array to be `present` on the device (OpenACC ``enter data`` in the
+driver, src/app/sources/main.f90:197-425), arrays must be mirrored first with
+:func:`copyin` / :func:`create`; results come back with :func:`update_host`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libmsod2d_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libmsod_host.so")
+
+# status codes (include/msod2d.h)
+MSOD_OK = 0
+MSOD_ERR_ARG = 1
+MSOD_ERR_NOT_PRESENT = 2
+MSOD_ERR_PORDER = 3
+MSOD_ERR_TABLE = 4
+MSOD_ERR_HIP = 5
+MSOD_ERR_STATE = 6
+MSOD_ERR_MESH = 7
+MSOD_ERR_MODEL = 8
+
+# timer ids of msod_timing_get
+TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP = range(6)
+
+# the C-ABI entry points declared in include/msod2d.h (checked by the tests)
+ABI_SYMBOLS = (
+    "msod_init", "msod_finalize", "msod_status_string", "msod_last_error",
+    "msod_data_copyin", "msod_data_create", "msod_data_update_device",
+    "msod_data_update_host", "msod_data_delete", "msod_data_present",
+    "msod_data_deviceptr", "msod_data_invalidate", "msod_set_async",
+    "msod_synchronize", "msod_timing_enable", "msod_timing_get", "msod_timing_reset",
+    "msod_full_convec_ijk", "msod_full_diffusion_ijk", "msod_set_diffusion_coefficients",
+    "msod_full_convec_ijk_mixed", "msod_full_diffusion_ijk_mixed",
+)
+
+
+class MsodError(RuntimeError):
+    """A non-zero status from the C ABI."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"msod status {status}: {message}")
+        self.status = status
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+_hip = None
+_host = None
+
+
+def _load(path: str, what: str) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise RuntimeError(f"{what} not built: {path} is missing (run __graft_entry__.build() "
+                           f"or `make -C minisod2d_amd`)")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+
+
+def hip_lib() -> ctypes.CDLL:
+    """The HIP kernel library (raises if it is not built)."""
+    global _hip
+    if _hip is None:
+        L = _load(HIP_LIB, "HIP kernel library")
+        L.msod_status_string.restype = ctypes.c_char_p
+        L.msod_last_error.restype = ctypes.c_char_p
+        L.msod_data_deviceptr.restype = _vp
+        L.msod_data_deviceptr.argtypes = [_vp]
+        for n in ("msod_data_copyin", "msod_data_create", "msod_data_update_device",
+                  "msod_data_update_host"):
+            getattr(L, n).argtypes = [_vp, ctypes.c_size_t]
+        for n in ("msod_data_delete", "msod_data_present", "msod_data_invalidate"):
+            getattr(L, n).argtypes = [_vp]
+        L.msod_timing_get.argtypes = [_i, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_int64)]
+        L.msod_set_diffusion_coefficients.argtypes = [_d, _d]
+        L.msod_full_convec_ijk.argtypes = [_i, _i, _i] + [_vp] * 19
+        L.msod_full_diffusion_ijk.argtypes = [_i, _i, _i] + [_vp] * 11 + [_d, _d] + [_vp] * 9
+        L.msod_full_convec_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 12 + [_d] + [_vp] * 3
+        L.msod_full_diffusion_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
+        _hip = L
+    return _hip
+
+
+def host_lib() -> ctypes.CDLL:
+    """The Fortran host library (tables, meshes, geometry, fields)."""
+    global _host
+    if _host is None:
+        L = _load(HOST_LIB, "Fortran host library")
+        L.msodh_box_mesh.argtypes = [_i, _i, _i, _i, _d, ctypes.c_int64, _vp, _vp]
+        L.msodh_elem_jacobian.argtypes = [_i, _i, _i] + [_vp] * 6
+        L.msodh_elem_jacobian.restype = _i
+        L.msodh_flow_fields.argtypes = [ctypes.c_int64, _vp, _d, _d, ctypes.c_int64, _d] + [_vp] * 7
+        L.msodh_viscosity_fields.argtypes = [ctypes.c_int64, _i, ctypes.c_int64, ctypes.c_int64,
+                                             _d, _d, _vp, _vp]
+        L.msodh_disjoint_mesh.argtypes = [_i, _i, _i, _vp, _vp, _vp]
+        _host = L
+    return _host
+
+
+def _check(status: int) -> None:
+    if status != MSOD_OK:
+        msg = hip_lib().msod_last_error().decode(errors="replace")
+        raise MsodError(status, msg)
+
+
+def _ptr(a) -> int | None:
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# device state and data region
+# ---------------------------------------------------------------------------
+def init(device: int = 0) -> None:
+    _check(hip_lib().msod_init(device))
+
+
+def finalize() -> None:
+    _check(hip_lib().msod_finalize())
+
+
+def copyin(*arrays: np.ndarray) -> None:
+    """``!$acc enter data copyin`` for each array."""
+    for a in arrays:
+        _check(hip_lib().msod_data_copyin(a.ctypes.data, a.nbytes))
+
+
+def create(*arrays: np.ndarray) -> None:
+    """``!$acc enter data create`` for each array (device copy zeroed)."""
+    for a in arrays:
+        _check(hip_lib().msod_data_create(a.ctypes.data, a.nbytes))
+
+
+def update_device(*arrays: np.ndarray) -> None:
+    for a in arrays:
+        _check(hip_lib().msod_data_update_device(a.ctypes.data, a.nbytes))
+
+
+def update_host(*arrays: np.ndarray) -> None:
+    for a in arrays:
+        _check(hip_lib().msod_data_update_host(a.ctypes.data, a.nbytes))
+
+
+def delete(*arrays: np.ndarray) -> None:
+    for a in arrays:
+        _check(hip_lib().msod_data_delete(a.ctypes.data))
+
+
+def present(a: np.ndarray) -> bool:
+    return bool(hip_lib().msod_data_present(a.ctypes.data))
+
+
+def set_async(flag: bool) -> None:
+    _check(hip_lib().msod_set_async(1 if flag else 0))
+
+
+def synchronize() -> None:
+    _check(hip_lib().msod_synchronize())
+
+
+def timing_enable(flag: bool) -> None:
+    _check(hip_lib().msod_timing_enable(1 if flag else 0))
+
+
+def timing_reset() -> None:
+    _check(hip_lib().msod_timing_reset())
+
+
+def timing_get(kernel: int) -> tuple[float, int]:
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    _check(hip_lib().msod_timing_get(kernel, ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def set_diffusion_coefficients(c_rho: float = 1.0, c_ener: float = 1.0) -> None:
+    _check(hip_lib().msod_set_diffusion_coefficients(c_rho, c_ener))
+
+
+# ---------------------------------------------------------------------------
+# the two kernels, reference argument lists
+# ---------------------------------------------------------------------------
+def _porder(invAtoIJK: np.ndarray) -> int:
+    return int(invAtoIJK.shape[0]) - 1
+
+
+def full_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK,
+                    gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener):
+    """Convective residual (elem_convec.f90:198-431), FP64, on the GPU."""
+    _check(hip_lib().msod_full_convec_ijk(
+        _porder(invAtoIJK), nelem, npoin, *map(_ptr, (
+            connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ,
+            gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener))))
+
+
+def full_diffusion_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK,
+                       gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs,
+                       Rmass, Rmom, Rener):
+    """Diffusive residual (elem_diffu.f90:10-197), FP64, on the GPU."""
+    L = hip_lib()
+    p = list(map(_ptr, (connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI,
+                        gmshAtoJ, gmshAtoK)))
+    r = list(map(_ptr, (rho, u, Tem, mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener)))
+    _check(L.msod_full_diffusion_ijk(_porder(invAtoIJK), nelem, npoin, *p, float(Cp), float(Pr), *r))
+
+
+def full_convec_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                          gmshAtoJ, gmshAtoK, u32, q32, rho32, E32, gamma_gas, Rmass, Rmom, Rener):
+    """Mixed-precision convective residual: FP32 state/geometry, FP64 assembly."""
+    _check(hip_lib().msod_full_convec_ijk_mixed(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
+                    u32, q32, rho32, E32)), float(gamma_gas), *map(_ptr, (Rmass, Rmom, Rener))))
+
+
+def full_diffusion_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                             gmshAtoJ, gmshAtoK, Cp, Pr, rho32, u32, Tem32, mu_fluid32, mu_e32,
+                             mu_sgs32, Rmass, Rmom, Rener):
+    """Mixed-precision diffusive residual: FP32 state/geometry, FP64 assembly."""
+    _check(hip_lib().msod_full_diffusion_ijk_mixed(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
+        float(Cp), float(Pr),
+        *map(_ptr, (rho32, u32, Tem32, mu_fluid32, mu_e32, mu_sgs32, Rmass, Rmom, Rener))))
+
+
+# ---------------------------------------------------------------------------
+# synthetic problems (Fortran host library)
+# ---------------------------------------------------------------------------
+GAMMA = 1.4
+RGAS = 287.0
+CP = 1004.5
+PR = 0.71
+SEED_RHO, SEED_MUE, SEED_MUSGS, SEED_JITTER = 1001, 1002, 1003, 1004
+
+
+def _f(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype, order="F")
+
+
+@dataclass
+class Tables:
+    porder: int
+    invAtoIJK: np.ndarray
+    AtoIJK: np.ndarray
+    AtoI: np.ndarray
+    AtoJ: np.ndarray
+    AtoK: np.ndarray
+    xgp: np.ndarray
+    wgp: np.ndarray
+    Ngp: np.ndarray
+    dNgp: np.ndarray
+    dlxigp_ip: np.ndarray
+
+    @property
+    def nnode(self) -> int:
+        return (self.porder + 1) ** 3
+
+
+def hex_tables(porder: int) -> Tables:
+    q = porder + 1
+    nn = q ** 3
+    t = [_f((q, q, q), np.int32), _f(nn, np.int32), _f(nn, np.int32), _f(nn, np.int32),
+         _f(nn, np.int32), _f((nn, 3)), _f(nn), _f((nn, nn)), _f((3, nn, nn)), _f((nn, 3, q))]
+    host_lib().msodh_hex_tables(porder, *[x.ctypes.data_as(_vp) for x in t])
+    return Tables(porder, *t)
+
+
+@dataclass
+class Problem:
+    """Inputs of both kernels for one mesh, in the reference layouts."""
+    tables: Tables
+    nelem: int
+    npoin: int
+    connec: np.ndarray
+    coord: np.ndarray
+    He: np.ndarray
+    gpvol: np.ndarray
+    u: np.ndarray
+    q: np.ndarray
+    rho: np.ndarray
+    pr: np.ndarray
+    E: np.ndarray
+    Tem: np.ndarray
+    mu_fluid: np.ndarray
+    mu_e: np.ndarray
+    mu_sgs: np.ndarray
+    Cp: float = CP
+    Pr: float = PR
+    gamma: float = GAMMA
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def porder(self) -> int:
+        return self.tables.porder
+
+    def convec_args(self, Rmass, Rmom, Rener):
+        t = self.tables
+        return (self.nelem, self.npoin, self.connec, t.Ngp, t.dNgp, self.He, self.gpvol,
+                t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, self.u, self.q,
+                self.rho, self.pr, self.E, Rmass, Rmom, Rener)
+
+    def diffu_args(self, Rmass, Rmom, Rener, mu_sgs=None):
+        t = self.tables
+        return (self.nelem, self.npoin, self.connec, t.Ngp, t.dNgp, self.He, self.gpvol,
+                t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, self.Cp, self.Pr,
+                self.rho, self.u, self.Tem, self.mu_fluid, self.mu_e,
+                self.mu_sgs if mu_sgs is None else mu_sgs, Rmass, Rmom, Rener)
+
+    def outputs(self):
+        return _f(self.npoin), _f((self.npoin, 3)), _f(self.npoin)
+
+    def device_inputs(self):
+        t = self.tables
+        return (self.connec, self.He, self.gpvol, self.u, self.q, self.rho, self.pr, self.E,
+                self.Tem, self.mu_fluid, self.mu_e, self.mu_sgs)
+
+    def fp32(self) -> dict:
+        """FP32 copies of geometry and state for the mixed-precision kernels."""
+        c = lambda a: np.asfortranarray(a, dtype=np.float32)
+        return dict(He=c(self.He), gpvol=c(self.gpvol), u=c(self.u), q=c(self.q),
+                    rho=c(self.rho), E=c(self.E), Tem=c(self.Tem), mu_fluid=c(self.mu_fluid),
+                    mu_e=c(self.mu_e), mu_sgs=c(self.mu_sgs))
+
+
+def box_problem(nx: int, porder: int, ny: int | None = None, nz: int | None = None,
+                jitter: float = 0.0, rho_jump: float = 0.0, threads: int = 0,
+                seeds=(SEED_RHO, SEED_MUE, SEED_MUSGS, SEED_JITTER)) -> Problem:
+    """Conforming [0,1]^3 hex mesh of nx*ny*nz elements of order porder with the
+    synthetic fields (see minisod2d_amd/fortran/msod_fields.f90)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    H = host_lib()
+    if threads:
+        H.msodh_set_threads(threads)
+    t = hex_tables(porder)
+    ne = ctypes.c_int64()
+    npn = ctypes.c_int64()
+    H.msodh_box_sizes(nx, ny, nz, porder, ctypes.byref(ne), ctypes.byref(npn))
+    nelem, npoin = ne.value, npn.value
+    if npoin >= 2 ** 31 or nelem * t.nnode >= 2 ** 31:
+        raise ValueError("mesh too large for the int32 index format of the reference")
+    nn = t.nnode
+    connec = _f((nelem, nn), np.int32)
+    coord = _f((npoin, 3))
+    H.msodh_box_mesh(nx, ny, nz, porder, jitter, seeds[3], connec.ctypes.data, coord.ctypes.data)
+    He = _f((3, 3, nn, nelem))
+    gpvol = _f((1, nn, nelem))
+    nbad = H.msodh_elem_jacobian(nelem, npoin, nn, connec.ctypes.data, coord.ctypes.data,
+                                 t.dNgp.ctypes.data, t.wgp.ctypes.data, gpvol.ctypes.data,
+                                 He.ctypes.data)
+    if nbad:
+        raise ValueError(f"{nbad} quadrature points with non-positive Jacobian")
+    u, q = _f((npoin, 3)), _f((npoin, 3))
+    rho, pr, E, Tem, muf = (_f(npoin) for _ in range(5))
+    H.msodh_flow_fields(npoin, coord.ctypes.data, GAMMA, RGAS, seeds[0], rho_jump,
+                        *[a.ctypes.data for a in (u, q, rho, pr, E, Tem, muf)])
+    mu_e, mu_sgs = _f((nelem, nn)), _f((nelem, nn))
+    H.msodh_viscosity_fields(nelem, nn, seeds[1], seeds[2], 1e-3, 1e-4, mu_e.ctypes.data,
+                             mu_sgs.ctypes.data)
+    return Problem(t, nelem, npoin, connec, coord, He, gpvol, u, q, rho, pr, E, Tem, muf, mu_e,
+                   mu_sgs, meta=dict(nx=nx, ny=ny, nz=nz, porder=porder, jitter=jitter,
+                                     rho_jump=rho_jump, seeds=list(seeds)))
+
+
+def reference_problem(nelem: int, porder: int) -> Problem:
+    """The reference driver's own case (src/app/sources/main.f90:257-413): the
+    disjoint mesh of mod_mesh.f90, fields of ones with spikes at local nodes
+    1 and 4 of every element, E and Tem from the driver's relations, Cp=Pr=1."""
+    H = host_lib()
+    t = hex_tables(porder)
+    nn = t.nnode
+    npoin = nelem * nn
+    connec = _f((nelem, nn), np.int32)
+    coord = _f((npoin, 3))
+    H.msodh_disjoint_mesh(nelem, npoin, nn, t.xgp.ctypes.data, connec.ctypes.data,
+                          coord.ctypes.data)
+    He = _f((3, 3, nn, nelem))
+    gpvol = _f((1, nn, nelem))
+    H.msodh_elem_jacobian(nelem, npoin, nn, connec.ctypes.data, coord.ctypes.data,
+                          t.dNgp.ctypes.data, t.wgp.ctypes.data, gpvol.ctypes.data, He.ctypes.data)
+    u = np.ones((npoin, 3), order="F")
+    rho = np.ones(npoin)
+    pr = np.ones(npoin)
+    i1 = connec[:, 0] - 1
+    i4 = connec[:, 3] - 1
+    u[i1, :] = 10.0
+    u[i4, :] = 20.0
+    rho[i1] = 10.0
+    rho[i4] = 20.0
+    pr[i1] = 100.0
+    pr[i4] = 200.0
+    q = np.asfortranarray(rho[:, None] * u)
+    fmag = (q * u).sum(axis=1)
+    eint = pr / ((GAMMA - 1.0) * rho)
+    E = rho * eint + 0.5 * fmag
+    Tem = pr / (rho * RGAS)
+    muf = np.ones(npoin)
+    mu_e = np.ones((nelem, nn), order="F")
+    mu_sgs = np.ones((nelem, nn), order="F")
+    return Problem(t, nelem, npoin, connec, coord, He, gpvol, u, q, rho, pr, E, Tem, muf, mu_e,
+                   mu_sgs, Cp=1.0, Pr=1.0, meta=dict(reference_case=True, nelem=nelem,
+                                                     porder=porder))

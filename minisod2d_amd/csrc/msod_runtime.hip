@@ -1,0 +1,930 @@
+// Runtime behind include/msod2d.h: device selection and stream, the data
+// region (host-address-keyed device mirrors, the OpenACC `enter data` model of
+// the reference driver, src/app/sources/main.f90:197-425), the element-major
+// layouts derived once per mesh, table validation and the kernel calls.
+#include "msod2d.h"
+#include "msod_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace msod {
+
+// ---------------------------------------------------------------------------
+// layout kernels
+// ---------------------------------------------------------------------------
+__global__ void k_zero_outputs(double *a, double *b, double *c, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 5 * n; i += stride) {
+        if (i < n)
+            a[i] = 0.0;
+        else if (i < 4 * n)
+            b[i - n] = 0.0;
+        else
+            c[i - 4 * n] = 0.0;
+    }
+}
+
+__global__ void k_count_nodes(const int32_t *connec, int64_t total, int64_t npoin, uint32_t *mult,
+                              int *bad)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int32_t v = connec[i];
+        if (v < 1 || (int64_t)v > npoin)
+            atomicAdd(bad, 1);
+        else
+            atomicAdd(mult + (v - 1), 1u);
+    }
+}
+
+__global__ void k_build_conn(const int32_t *connec, int64_t nelem, int nn, const int32_t *l2g,
+                             const uint32_t *mult, uint32_t *conn)
+{
+    const int64_t total = nelem * nn;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t e = i / nn;
+        const int l = (int)(i - e * nn);
+        const uint32_t g = (uint32_t)(connec[e + nelem * l2g[l]] - 1);
+        conn[i] = g | (mult[g] == 1u ? 0x80000000u : 0u);
+    }
+}
+
+template <typename T>
+__global__ void k_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn, const int32_t *l2g,
+                            T *geo, T *gvol)
+{
+    const int64_t total = nelem * nn;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t e = i / nn;
+        const int l = (int)(i - e * nn);
+        const int64_t a = l2g[l];
+        const T *src = He + 9 * (a + (int64_t)nn * e);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) geo[(e * 9 + c) * nn + l] = src[c];
+        gvol[i] = gpvol[a + (int64_t)nn * e];
+    }
+}
+
+static unsigned grid_for(int64_t n, int bs)
+{
+    int64_t g = (n + bs - 1) / bs;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_outputs, dim3(grid_for(5 * n, 256)), dim3(256), 0, s, Rmass, Rmom,
+                       Rener, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_conn(const int32_t *connec, int64_t nelem, int nn, int64_t npoin,
+                             const int32_t *l2g, uint32_t *conn, uint32_t *mult, int *bad,
+                             int *nbad, hipStream_t s)
+{
+    hipError_t err = hipMemsetAsync(mult, 0, sizeof(uint32_t) * npoin, s);
+    if (err != hipSuccess) return err;
+    err = hipMemsetAsync(bad, 0, sizeof(int), s);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_count_nodes, dim3(grid_for(nelem * nn, 256)), dim3(256), 0, s, connec,
+                       nelem * nn, npoin, mult, bad);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    int hbad = 0;
+    err = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (err != hipSuccess) return err;
+    err = hipStreamSynchronize(s);
+    if (err != hipSuccess) return err;
+    *nbad = hbad;
+    if (hbad != 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_conn, dim3(grid_for(nelem * nn, 256)), dim3(256), 0, s, connec,
+                       nelem, nn, l2g, mult, conn);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
+                            const int32_t *l2g, T *geo, T *gvol, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_build_geo<T>), dim3(grid_for(nelem * nn, 256)), dim3(256), 0, s, He,
+                       gpvol, nelem, nn, l2g, geo, gvol);
+    return hipGetLastError();
+}
+template hipError_t launch_build_geo<double>(const double *, const double *, int64_t, int,
+                                             const int32_t *, double *, double *, hipStream_t);
+template hipError_t launch_build_geo<float>(const float *, const float *, int64_t, int,
+                                            const int32_t *, float *, float *, hipStream_t);
+
+}  // namespace msod
+
+// ---------------------------------------------------------------------------
+// runtime state
+// ---------------------------------------------------------------------------
+namespace {
+
+using namespace msod;
+
+struct Region {
+    size_t bytes;
+    void *dev;
+    uint64_t version;
+};
+
+struct TableSet {
+    int porder;
+    std::vector<double> D;       // q*q
+    std::vector<int32_t> l2g;    // nn
+    double *dD = nullptr;
+    int32_t *dl2g = nullptr;
+};
+
+struct DerivedConn {
+    const void *src;
+    uint64_t ver;
+    int64_t nelem, npoin;
+    int porder;
+    std::vector<int32_t> l2g;
+    uint32_t *conn;
+};
+
+struct DerivedGeo {
+    const void *srcHe, *srcVol;
+    uint64_t verHe, verVol;
+    int64_t nelem;
+    int porder;
+    size_t elt;
+    std::vector<int32_t> l2g;
+    void *geo, *gvol;
+};
+
+constexpr int kNumTimers = 6;
+struct Pending {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+struct State {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool async = false;
+    bool timing = false;
+    double c_rho = 1.0, c_ener = 1.0;
+    std::map<uintptr_t, Region> present;
+    uint64_t next_version = 1;
+    std::vector<TableSet *> tables;
+    std::vector<DerivedConn> conns;
+    std::vector<DerivedGeo> geos;
+    int *d_bad = nullptr;
+    double ms[kNumTimers] = {0};
+    int64_t launches[kNumTimers] = {0};
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+};
+
+State g;
+std::recursive_mutex g_mu;
+thread_local std::string g_err;
+
+int fail(int status, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return status;
+}
+
+int hipfail(hipError_t e, const char *what)
+{
+    return fail(MSOD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(call, what)                          \
+    do {                                            \
+        hipError_t e_ = (call);                     \
+        if (e_ != hipSuccess) return hipfail(e_, what); \
+    } while (0)
+
+#define NEED_INIT()                                                         \
+    do {                                                                    \
+        if (!g.init) return fail(MSOD_ERR_STATE, "msod_init has not been called"); \
+    } while (0)
+
+// Present-table lookup by containment.
+Region *find_region(const void *p, uintptr_t *base)
+{
+    if (g.present.empty()) return nullptr;
+    auto it = g.present.upper_bound((uintptr_t)p);
+    if (it == g.present.begin()) return nullptr;
+    --it;
+    if ((uintptr_t)p < it->first + it->second.bytes) {
+        *base = it->first;
+        return &it->second;
+    }
+    return nullptr;
+}
+
+bool is_device_pointer(const void *p)
+{
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice;
+}
+
+// Host or device address of an array argument -> device address.
+int resolve(const void *p, size_t need, const char *name, const void **dev, uint64_t *ver)
+{
+    if (!p) return fail(MSOD_ERR_ARG, "%s is NULL", name);
+    uintptr_t base = 0;
+    if (Region *r = find_region(p, &base)) {
+        const size_t off = (uintptr_t)p - base;
+        if (off + need > r->bytes)
+            return fail(MSOD_ERR_MESH, "%s: present array holds %zu bytes past this address, %zu needed",
+                        name, r->bytes - off, need);
+        *dev = (const char *)r->dev + off;
+        *ver = r->version;
+        return MSOD_OK;
+    }
+    if (is_device_pointer(p)) {
+        *dev = p;
+        *ver = 0;
+        return MSOD_OK;
+    }
+    return fail(MSOD_ERR_NOT_PRESENT, "%s (%p) is not present on the device", name, p);
+}
+
+// Small tables are read on the host: directly when the address is host
+// memory (present or not), through a copy when it is device memory.
+int host_view(const void *p, size_t bytes, std::vector<char> &tmp, const char **out, const char *name)
+{
+    if (!p) return fail(MSOD_ERR_ARG, "%s is NULL", name);
+    if (is_device_pointer(p)) {
+        tmp.resize(bytes);
+        HIPCHK(hipMemcpy(tmp.data(), p, bytes, hipMemcpyDeviceToHost), "reading a device table");
+        *out = tmp.data();
+    } else {
+        *out = (const char *)p;
+    }
+    return MSOD_OK;
+}
+
+// Checks invAtoIJK/AtoI/J/K/dlxigp_ip and returns the cached device table set.
+int get_tables(int porder, const int32_t *invAtoIJK, const int32_t *AtoI, const int32_t *AtoJ,
+               const int32_t *AtoK, const double *dlxigp_ip, TableSet **out)
+{
+    const int q = porder + 1, nn = q * q * q;
+    std::vector<char> t0, t1, t2, t3, t4;
+    const char *pinv, *pi, *pj, *pk, *pdl;
+    int st;
+    if ((st = host_view(invAtoIJK, sizeof(int32_t) * nn, t0, &pinv, "invAtoIJK"))) return st;
+    if ((st = host_view(AtoI, sizeof(int32_t) * nn, t1, &pi, "gmshAtoI"))) return st;
+    if ((st = host_view(AtoJ, sizeof(int32_t) * nn, t2, &pj, "gmshAtoJ"))) return st;
+    if ((st = host_view(AtoK, sizeof(int32_t) * nn, t3, &pk, "gmshAtoK"))) return st;
+    if ((st = host_view(dlxigp_ip, sizeof(double) * nn * 3 * q, t4, &pdl, "dlxigp_ip"))) return st;
+    const int32_t *inv = (const int32_t *)pinv, *ai = (const int32_t *)pi,
+                  *aj = (const int32_t *)pj, *ak = (const int32_t *)pk;
+    const double *dl = (const double *)pdl;
+
+    std::vector<int32_t> l2g(nn);
+    std::vector<char> seen(nn, 0);
+    for (int l = 0; l < nn; ++l) {
+        const int32_t a = inv[l];
+        if (a < 1 || a > nn || seen[a - 1])
+            return fail(MSOD_ERR_TABLE, "invAtoIJK is not a permutation of 1..%d", nn);
+        seen[a - 1] = 1;
+        l2g[l] = a - 1;
+    }
+    for (int a = 0; a < nn; ++a) {
+        const int i = ai[a], j = aj[a], k = ak[a];
+        if (i < 1 || i > q || j < 1 || j > q || k < 1 || k > q ||
+            inv[(i - 1) + q * ((j - 1) + q * (k - 1))] != a + 1)
+            return fail(MSOD_ERR_TABLE, "gmshAtoI/J/K disagree with invAtoIJK at node %d", a + 1);
+    }
+    // D[i][n] from the nodes on the xi-line (i,1,1); every node/direction
+    // must carry the same row of the 1-D derivative (collocation at GLL points)
+    std::vector<double> D(q * q);
+    for (int i = 0; i < q; ++i) {
+        const int a = inv[i] - 1;
+        for (int n = 0; n < q; ++n) D[i * q + n] = dl[a + nn * (0 + 3 * n)];
+    }
+    for (int a = 0; a < nn; ++a) {
+        const int idx[3] = {ai[a] - 1, aj[a] - 1, ak[a] - 1};
+        for (int d = 0; d < 3; ++d)
+            for (int n = 0; n < q; ++n)
+                if (std::memcmp(&dl[a + nn * (d + 3 * n)], &D[idx[d] * q + n], sizeof(double)) != 0)
+                    return fail(MSOD_ERR_TABLE,
+                                "dlxigp_ip(%d,%d,%d) is not the 1-D derivative row of its node", a + 1,
+                                d + 1, n + 1);
+    }
+    for (TableSet *ts : g.tables)
+        if (ts->porder == porder && ts->l2g == l2g &&
+            std::memcmp(ts->D.data(), D.data(), sizeof(double) * q * q) == 0) {
+            *out = ts;
+            return MSOD_OK;
+        }
+    TableSet *ts = new TableSet;
+    ts->porder = porder;
+    ts->D = D;
+    ts->l2g = l2g;
+    HIPCHK(hipMalloc(&ts->dD, sizeof(double) * q * q), "hipMalloc tables");
+    HIPCHK(hipMalloc(&ts->dl2g, sizeof(int32_t) * nn), "hipMalloc tables");
+    HIPCHK(hipMemcpy(ts->dD, D.data(), sizeof(double) * q * q, hipMemcpyHostToDevice), "upload D");
+    HIPCHK(hipMemcpy(ts->dl2g, l2g.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice), "upload l2g");
+    g.tables.push_back(ts);
+    *out = ts;
+    return MSOD_OK;
+}
+
+constexpr size_t kMaxDerived = 4;
+
+int get_conn(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, const TableSet *ts,
+             const uint32_t **out)
+{
+    for (auto &c : g.conns)
+        if (c.src == dconnec && c.ver == ver && c.nelem == nelem && c.npoin == npoin &&
+            c.porder == ts->porder && c.l2g == ts->l2g) {
+            *out = c.conn;
+            return MSOD_OK;
+        }
+    const int nn = (int)ts->l2g.size();
+    DerivedConn c{dconnec, ver, nelem, npoin, ts->porder, ts->l2g, nullptr};
+    uint32_t *mult = nullptr;
+    HIPCHK(hipMalloc(&c.conn, sizeof(uint32_t) * nelem * nn), "hipMalloc element-major connec");
+    HIPCHK(hipMalloc(&mult, sizeof(uint32_t) * npoin), "hipMalloc node counts");
+    int nbad = 0;
+    hipError_t e = launch_build_conn((const int32_t *)dconnec, nelem, nn, npoin, ts->dl2g, c.conn,
+                                     mult, g.d_bad, &nbad, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    (void)hipFree(mult);
+    if (e == hipSuccess && nbad != 0) {
+        (void)hipFree(c.conn);
+        return fail(MSOD_ERR_MESH, "connec holds node numbers outside 1..npoin (npoin=%lld)",
+                    (long long)npoin);
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(c.conn);
+        return hipfail(e, "building element-major connec");
+    }
+    if (g.conns.size() >= kMaxDerived) {
+        (void)hipFree(g.conns.front().conn);
+        g.conns.erase(g.conns.begin());
+    }
+    g.conns.push_back(c);
+    *out = c.conn;
+    return MSOD_OK;
+}
+
+template <typename T>
+int get_geo(const void *dHe, uint64_t vHe, const void *dVol, uint64_t vVol, int64_t nelem,
+            const TableSet *ts, const T **geo, const T **gvol)
+{
+    for (auto &c : g.geos)
+        if (c.srcHe == dHe && c.srcVol == dVol && c.verHe == vHe && c.verVol == vVol &&
+            c.nelem == nelem && c.porder == ts->porder && c.elt == sizeof(T) && c.l2g == ts->l2g) {
+            *geo = (const T *)c.geo;
+            *gvol = (const T *)c.gvol;
+            return MSOD_OK;
+        }
+    const int nn = (int)ts->l2g.size();
+    DerivedGeo c{dHe, dVol, vHe, vVol, nelem, ts->porder, sizeof(T), ts->l2g, nullptr, nullptr};
+    HIPCHK(hipMalloc(&c.geo, sizeof(T) * 9 * nelem * nn), "hipMalloc element-major He");
+    HIPCHK(hipMalloc(&c.gvol, sizeof(T) * nelem * nn), "hipMalloc element-major gpvol");
+    hipError_t e = launch_build_geo<T>((const T *)dHe, (const T *)dVol, nelem, nn, ts->dl2g,
+                                       (T *)c.geo, (T *)c.gvol, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) {
+        (void)hipFree(c.geo);
+        (void)hipFree(c.gvol);
+        return hipfail(e, "building element-major geometry");
+    }
+    if (g.geos.size() >= kMaxDerived) {
+        (void)hipFree(g.geos.front().geo);
+        (void)hipFree(g.geos.front().gvol);
+        g.geos.erase(g.geos.begin());
+    }
+    g.geos.push_back(c);
+    *geo = (const T *)c.geo;
+    *gvol = (const T *)c.gvol;
+    return MSOD_OK;
+}
+
+void drop_derived_in(uintptr_t lo, uintptr_t hi)
+{
+    auto inside = [&](const void *p) { return (uintptr_t)p >= lo && (uintptr_t)p < hi; };
+    for (size_t i = g.conns.size(); i-- > 0;)
+        if (inside(g.conns[i].src)) {
+            (void)hipFree(g.conns[i].conn);
+            g.conns.erase(g.conns.begin() + i);
+        }
+    for (size_t i = g.geos.size(); i-- > 0;)
+        if (inside(g.geos[i].srcHe) || inside(g.geos[i].srcVol)) {
+            (void)hipFree(g.geos[i].geo);
+            (void)hipFree(g.geos[i].gvol);
+            g.geos.erase(g.geos.begin() + i);
+        }
+}
+
+// --- timing ---------------------------------------------------------------
+hipEvent_t take_event()
+{
+    if (!g.free_events.empty()) {
+        hipEvent_t e = g.free_events.back();
+        g.free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct TimedLaunch {
+    int kernel;
+    hipEvent_t a = nullptr;
+    explicit TimedLaunch(int k) : kernel(k)
+    {
+        if (g.timing) {
+            a = take_event();
+            (void)hipEventRecord(a, g.stream);
+        }
+    }
+    void done()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event();
+        (void)hipEventRecord(b, g.stream);
+        g.pending.push_back({kernel, a, b});
+        a = nullptr;
+    }
+};
+
+void collect_timing()
+{
+    for (auto &p : g.pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            g.ms[p.kernel] += ms;
+            g.launches[p.kernel] += 1;
+        }
+        g.free_events.push_back(p.a);
+        g.free_events.push_back(p.b);
+    }
+    g.pending.clear();
+}
+
+int finish_call(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hipfail(e, what);
+    if (!g.async) {
+        e = hipStreamSynchronize(g.stream);
+        if (e != hipSuccess) return hipfail(e, what);
+    }
+    if (g.pending.size() > 4096) collect_timing();
+    return MSOD_OK;
+}
+
+int check_sizes(int porder, int nelem, int npoin)
+{
+    if (porder < 1 || porder > kMaxP)
+        return fail(MSOD_ERR_PORDER, "porder=%d: kernels are compiled for 1..%d", porder, kMaxP);
+    if (nelem <= 0 || npoin <= 0) return fail(MSOD_ERR_ARG, "nelem=%d npoin=%d", nelem, npoin);
+    return MSOD_OK;
+}
+
+// Shared front half of the convective calls.
+template <typename T>
+int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const T *He,
+                  const T *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, const T *u,
+                  const T *q, const T *rho, const T *pr, const T *E, double gamma_gas,
+                  double *Rmass, double *Rmom, double *Rener, int timer)
+{
+    NEED_INIT();
+    int st = check_sizes(porder, nelem, npoin);
+    if (st) return st;
+    const int qn = porder + 1;
+    const int64_t nn = (int64_t)qn * qn * qn, NE = nelem, NP = npoin;
+    TableSet *ts;
+    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts))) return st;
+    const void *dc, *dHe, *dV, *du, *dq, *dr, *dp = nullptr, *dE, *dRm, *dRmo, *dRe;
+    uint64_t vc, vHe, vV, vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
+    if ((st = resolve(He, sizeof(T) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
+    if ((st = resolve(gpvol, sizeof(T) * NE * nn, "gpvol", &dV, &vV))) return st;
+    if ((st = resolve(u, sizeof(T) * 3 * NP, "u", &du, &vx))) return st;
+    if ((st = resolve(q, sizeof(T) * 3 * NP, "q", &dq, &vx))) return st;
+    if ((st = resolve(rho, sizeof(T) * NP, "rho", &dr, &vx))) return st;
+    if (pr && (st = resolve(pr, sizeof(T) * NP, "pr", &dp, &vx))) return st;
+    if ((st = resolve(E, sizeof(T) * NP, "E", &dE, &vx))) return st;
+    if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
+    if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
+    const uint32_t *conn;
+    const T *geo, *gvol;
+    if ((st = get_conn(dc, vc, NE, NP, ts, &conn))) return st;
+    if ((st = get_geo<T>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+
+    ConvecArgs<T> a;
+    a.nelem = NE;
+    a.npoin = NP;
+    a.conn = conn;
+    a.geo = geo;
+    a.gvol = gvol;
+    a.u = (const T *)du;
+    a.q = (const T *)dq;
+    a.rho = (const T *)dr;
+    a.pr = (const T *)dp;
+    a.E = (const T *)dE;
+    a.gamma_m1 = gamma_gas - 1.0;
+    a.Rmass = (double *)dRm;
+    a.Rmom = (double *)dRmo;
+    a.Rener = (double *)dRe;
+    a.tab = Tables{ts->dD, ts->dl2g};
+    {
+        TimedLaunch tz(0);
+        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
+        tz.done();
+    }
+    {
+        TimedLaunch tk(timer);
+        HIPCHK(launch_convec<T>(porder, a, g.stream), "convection kernel launch");
+        tk.done();
+    }
+    return finish_call("convection kernel");
+}
+
+template <typename T>
+int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const T *He,
+                 const T *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                 const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, double Cp,
+                 double Pr, const T *rho, const T *u, const T *Tem, const T *mu_fluid,
+                 const T *mu_e, const T *mu_sgs, double *Rmass, double *Rmom, double *Rener,
+                 int timer)
+{
+    NEED_INIT();
+    int st = check_sizes(porder, nelem, npoin);
+    if (st) return st;
+    const int qn = porder + 1;
+    const int64_t nn = (int64_t)qn * qn * qn, NE = nelem, NP = npoin;
+    TableSet *ts;
+    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts))) return st;
+    const void *dc, *dHe, *dV, *dr, *du, *dT, *dmf, *dme, *dms, *dRm, *dRmo, *dRe;
+    uint64_t vc, vHe, vV, vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
+    if ((st = resolve(He, sizeof(T) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
+    if ((st = resolve(gpvol, sizeof(T) * NE * nn, "gpvol", &dV, &vV))) return st;
+    if ((st = resolve(rho, sizeof(T) * NP, "rho", &dr, &vx))) return st;
+    if ((st = resolve(u, sizeof(T) * 3 * NP, "u", &du, &vx))) return st;
+    if ((st = resolve(Tem, sizeof(T) * NP, "Tem", &dT, &vx))) return st;
+    if ((st = resolve(mu_fluid, sizeof(T) * NP, "mu_fluid", &dmf, &vx))) return st;
+    if ((st = resolve(mu_e, sizeof(T) * NE * nn, "mu_e", &dme, &vx))) return st;
+    if ((st = resolve(mu_sgs, sizeof(T) * NE * nn, "mu_sgs", &dms, &vx))) return st;
+    if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
+    if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
+    const uint32_t *conn;
+    const T *geo, *gvol;
+    if ((st = get_conn(dc, vc, NE, NP, ts, &conn))) return st;
+    if ((st = get_geo<T>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+
+    DiffuArgs<T> a;
+    a.nelem = NE;
+    a.npoin = NP;
+    a.conn = conn;
+    a.geo = geo;
+    a.gvol = gvol;
+    a.rho = (const T *)dr;
+    a.u = (const T *)du;
+    a.Tem = (const T *)dT;
+    a.mu_fluid = (const T *)dmf;
+    a.mu_e = (const T *)dme;
+    a.mu_sgs = (const T *)dms;
+    a.Cp = Cp;
+    a.Pr = Pr;
+    a.c_rho = g.c_rho;
+    a.c_ener = g.c_ener;
+    a.Rmass = (double *)dRm;
+    a.Rmom = (double *)dRmo;
+    a.Rener = (double *)dRe;
+    a.tab = Tables{ts->dD, ts->dl2g};
+    {
+        TimedLaunch tz(0);
+        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
+        tz.done();
+    }
+    {
+        TimedLaunch tk(timer);
+        HIPCHK(launch_diffu<T>(porder, a, g.stream), "diffusion kernel launch");
+        tk.done();
+    }
+    return finish_call("diffusion kernel");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int msod_init(int device)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (g.init) {
+        if (device == g.device) return MSOD_OK;
+        return fail(MSOD_ERR_STATE, "already initialised on device %d", g.device);
+    }
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) return fail(MSOD_ERR_ARG, "device %d of %d", device, n);
+    HIPCHK(hipSetDevice(device), "hipSetDevice");
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking), "hipStreamCreate");
+    HIPCHK(hipMalloc(&g.d_bad, sizeof(int)), "hipMalloc");
+    g.device = device;
+    g.init = true;
+    return MSOD_OK;
+}
+
+int msod_finalize(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!g.init) return MSOD_OK;
+    (void)hipStreamSynchronize(g.stream);
+    collect_timing();
+    for (auto &kv : g.present) (void)hipFree(kv.second.dev);
+    g.present.clear();
+    drop_derived_in(0, UINTPTR_MAX);
+    for (TableSet *ts : g.tables) {
+        (void)hipFree(ts->dD);
+        (void)hipFree(ts->dl2g);
+        delete ts;
+    }
+    g.tables.clear();
+    for (hipEvent_t e : g.free_events) (void)hipEventDestroy(e);
+    g.free_events.clear();
+    (void)hipFree(g.d_bad);
+    (void)hipStreamDestroy(g.stream);
+    g = State{};
+    return MSOD_OK;
+}
+
+const char *msod_status_string(int status)
+{
+    switch (status) {
+    case MSOD_OK: return "ok";
+    case MSOD_ERR_ARG: return "invalid argument";
+    case MSOD_ERR_NOT_PRESENT: return "array not present on the device";
+    case MSOD_ERR_PORDER: return "porder not compiled";
+    case MSOD_ERR_TABLE: return "inconsistent element tables";
+    case MSOD_ERR_HIP: return "HIP runtime error";
+    case MSOD_ERR_STATE: return "library not initialised";
+    case MSOD_ERR_MESH: return "invalid mesh arrays";
+    case MSOD_ERR_MODEL: return "invalid closure model";
+    default: return "unknown status";
+    }
+}
+
+const char *msod_last_error(void) { return g_err.c_str(); }
+
+int msod_data_create(const void *host, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (!host || bytes == 0) return fail(MSOD_ERR_ARG, "data_create: NULL or empty");
+    uintptr_t base;
+    if (Region *r = find_region(host, &base)) {
+        if (base == (uintptr_t)host && r->bytes == bytes) return MSOD_OK;  // already present
+        return fail(MSOD_ERR_ARG, "data_create: %p overlaps a present array", host);
+    }
+    auto it = g.present.lower_bound((uintptr_t)host);
+    if (it != g.present.end() && it->first < (uintptr_t)host + bytes)
+        return fail(MSOD_ERR_ARG, "data_create: %p+%zu overlaps a present array", host, bytes);
+    void *dev = nullptr;
+    HIPCHK(hipMalloc(&dev, bytes), "hipMalloc");
+    hipError_t e = hipMemsetAsync(dev, 0, bytes, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return hipfail(e, "data_create");
+    }
+    g.present[(uintptr_t)host] = Region{bytes, dev, g.next_version++};
+    return MSOD_OK;
+}
+
+int msod_data_copyin(const void *host, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    uintptr_t base;
+    Region *r0 = host ? find_region(host, &base) : nullptr;
+    if (r0 && base == (uintptr_t)host && r0->bytes == bytes) return MSOD_OK;  // present: no copy
+    int st = msod_data_create(host, bytes);
+    if (st) return st;
+    Region &r = g.present[(uintptr_t)host];
+    HIPCHK(hipMemcpy(r.dev, host, bytes, hipMemcpyHostToDevice), "data_copyin");
+    return MSOD_OK;
+}
+
+int msod_data_update_device(const void *host, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    uintptr_t base;
+    Region *r = host ? find_region(host, &base) : nullptr;
+    if (!r) return fail(MSOD_ERR_NOT_PRESENT, "update_device: %p not present", host);
+    const size_t off = (uintptr_t)host - base;
+    if (off + bytes > r->bytes) return fail(MSOD_ERR_ARG, "update_device: range exceeds the array");
+    HIPCHK(hipStreamSynchronize(g.stream), "update_device");
+    HIPCHK(hipMemcpy((char *)r->dev + off, host, bytes, hipMemcpyHostToDevice), "update_device");
+    r->version = g.next_version++;
+    return MSOD_OK;
+}
+
+int msod_data_update_host(void *host, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    uintptr_t base;
+    Region *r = host ? find_region(host, &base) : nullptr;
+    if (!r) return fail(MSOD_ERR_NOT_PRESENT, "update_host: %p not present", host);
+    const size_t off = (uintptr_t)host - base;
+    if (off + bytes > r->bytes) return fail(MSOD_ERR_ARG, "update_host: range exceeds the array");
+    HIPCHK(hipStreamSynchronize(g.stream), "update_host");
+    HIPCHK(hipMemcpy(host, (const char *)r->dev + off, bytes, hipMemcpyDeviceToHost), "update_host");
+    return MSOD_OK;
+}
+
+int msod_data_delete(const void *host)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    auto it = g.present.find((uintptr_t)host);
+    if (it == g.present.end()) return fail(MSOD_ERR_NOT_PRESENT, "delete: %p not present", host);
+    HIPCHK(hipStreamSynchronize(g.stream), "delete");
+    const uintptr_t lo = (uintptr_t)it->second.dev, hi = lo + it->second.bytes;
+    drop_derived_in(lo, hi);
+    HIPCHK(hipFree(it->second.dev), "hipFree");
+    g.present.erase(it);
+    return MSOD_OK;
+}
+
+int msod_data_present(const void *host)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    uintptr_t base;
+    return (g.init && host && find_region(host, &base)) ? 1 : 0;
+}
+
+void *msod_data_deviceptr(const void *host)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    uintptr_t base;
+    Region *r = (g.init && host) ? find_region(host, &base) : nullptr;
+    return r ? (char *)r->dev + ((uintptr_t)host - base) : nullptr;
+}
+
+int msod_data_invalidate(const void *ptr)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    uintptr_t base;
+    if (Region *r = find_region(ptr, &base)) {
+        r->version = g.next_version++;
+        return MSOD_OK;
+    }
+    HIPCHK(hipStreamSynchronize(g.stream), "invalidate");
+    drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+    return MSOD_OK;
+}
+
+int msod_set_async(int async)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    g.async = async != 0;
+    return MSOD_OK;
+}
+
+int msod_synchronize(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    HIPCHK(hipStreamSynchronize(g.stream), "synchronize");
+    return MSOD_OK;
+}
+
+int msod_timing_enable(int enable)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    g.timing = enable != 0;
+    return MSOD_OK;
+}
+
+int msod_timing_get(int kernel, double *total_ms, int64_t *launches)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (kernel < 0 || kernel >= kNumTimers) return fail(MSOD_ERR_ARG, "timer %d", kernel);
+    collect_timing();
+    if (total_ms) *total_ms = g.ms[kernel];
+    if (launches) *launches = g.launches[kernel];
+    return MSOD_OK;
+}
+
+int msod_timing_reset(void)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    collect_timing();
+    for (int i = 0; i < kNumTimers; ++i) {
+        g.ms[i] = 0.0;
+        g.launches[i] = 0;
+    }
+    return MSOD_OK;
+}
+
+int msod_set_diffusion_coefficients(double c_rho, double c_ener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    g.c_rho = c_rho;
+    g.c_ener = c_ener;
+    return MSOD_OK;
+}
+
+int msod_full_convec_ijk(int porder, int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                         const double *dNgp, const double *He, const double *gpvol,
+                         const double *dlxigp_ip, const double *xgp, const int32_t *invAtoIJK,
+                         const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                         const double *u, const double *q, const double *rho, const double *pr,
+                         const double *E, double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    (void)Ngp; (void)dNgp; (void)xgp;
+    if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
+    return convec_common<double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, 0.0, Rmass, Rmom,
+                                 Rener, 1);
+}
+
+int msod_full_convec_ijk_mixed(int porder, int nelem, int npoin, const int32_t *connec,
+                               const float *He, const float *gpvol, const double *dlxigp_ip,
+                               const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                               const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const float *u,
+                               const float *q, const float *rho, const float *E, double gamma_gas,
+                               double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
+    return convec_common<float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
+                                Rmass, Rmom, Rener, 3);
+}
+
+int msod_full_diffusion_ijk(int porder, int nelem, int npoin, const int32_t *connec,
+                            const double *Ngp, const double *dNgp, const double *He,
+                            const double *gpvol, const double *dlxigp_ip, const double *xgp,
+                            const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                            const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp, double Pr,
+                            const double *rho, const double *u, const double *Tem,
+                            const double *mu_fluid, const double *mu_e, const double *mu_sgs,
+                            double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    (void)Ngp; (void)dNgp; (void)xgp;
+    return diffu_common<double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
+                                mu_sgs, Rmass, Rmom, Rener, 2);
+}
+
+int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_t *connec,
+                                  const float *He, const float *gpvol, const double *dlxigp_ip,
+                                  const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                                  const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                                  double Pr, const float *rho, const float *u, const float *Tem,
+                                  const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                  double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    return diffu_common<float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                               gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
+                               mu_sgs, Rmass, Rmom, Rener, 4);
+}
+
+}  // extern "C"

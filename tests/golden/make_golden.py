@@ -1,0 +1,83 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own kernels.
+
+Run in a container where /root/reference exists, after `make -C oracle ref`
+(which compiles the reference's Fortran sources, rp=8, one library per
+porder, into oracle/_ref/):
+
+    python tests/golden/make_golden.py
+
+Each fixture holds the reference's OUTPUTS for inputs produced by this
+repository's own deterministic host generator (minisod2d_amd.box_problem /
+reference_problem), plus a SHA-256 of every input array so that a test can
+tell a generator change from a kernel change.  Fixtures:
+
+  tables_pP.npz   reference element tables for porder P (main.f90:110-255)
+  jac_p4.npz      He, gpvol of elem_jacobian on a jittered 2x2x2 p=4 box
+  case_*.npz      Rmass/Rmom of full_convec_ijk, Rener of full_convec_ijk_split,
+                  Dmass/Dmom/Dener of full_diffusion_ijk
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import minisod2d_amd as m  # noqa: E402
+from oracle_lib import RefLib  # noqa: E402
+
+# (name, builder) -- small enough for the reference to finish in seconds
+CASES = {
+    "case_box8_p3": lambda: m.box_problem(8, 3),                                  # 8^3 elements, porder 3
+    "case_box2_p2_jit": lambda: m.box_problem(2, 2, jitter=0.05),
+    "case_box2_p3_jit": lambda: m.box_problem(2, 3, jitter=0.05),
+    "case_box3x2x2_p4_jit": lambda: m.box_problem(3, 4, ny=2, nz=2, jitter=0.05),
+    "case_box2_p5_jit": lambda: m.box_problem(2, 5, jitter=0.05),
+    "case_box2_p6_jit": lambda: m.box_problem(2, 6, jitter=0.05),
+    "case_box2_p7_jit_jump": lambda: m.box_problem(2, 7, jitter=0.05, rho_jump=0.1),
+    "case_box2_p8_jit": lambda: m.box_problem(2, 8, jitter=0.05),
+    "case_refdriver_p4": lambda: m.reference_problem(6, 4),
+    "case_refdriver_p3": lambda: m.reference_problem(5, 3),
+}
+
+INPUT_FIELDS = ("connec", "He", "gpvol", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")
+
+
+def digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(np.asfortranarray(a).ravel(order="F")).tobytes()).hexdigest()
+
+
+def input_digests(prob) -> dict:
+    return {k: digest(getattr(prob, k)) for k in INPUT_FIELDS}
+
+
+def main():
+    for p in range(2, 9):
+        t = RefLib(p).tables()
+        np.savez_compressed(os.path.join(HERE, f"tables_p{p}.npz"),
+                            **{k: v for k, v in t.items() if k not in ("Ngp", "dNgp")},
+                            Ngp_sha=digest(t["Ngp"]), dNgp_sha=digest(t["dNgp"]))
+    prob = m.box_problem(2, 4, jitter=0.05)
+    ref = RefLib(4)
+    He, gp = ref.jacobian(prob.nelem, prob.npoin, prob.connec, prob.coord, prob.tables.dNgp, prob.tables.wgp)
+    np.savez_compressed(os.path.join(HERE, "jac_p4.npz"), He=He, gpvol=gp, coord_sha=digest(prob.coord),
+                        connec_sha=digest(prob.connec))
+    for name, build in CASES.items():
+        prob = build()
+        ref = RefLib(prob.porder)
+        Rmass, Rmom, Rener, _ = ref.convec(prob)
+        Dmass, Dmom, Dener = ref.diffu(prob)
+        meta = dict(prob.meta, nelem=prob.nelem, npoin=prob.npoin, Cp=prob.Cp, Pr=prob.Pr,
+                    inputs=input_digests(prob))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), Rmass=Rmass, Rmom=Rmom, Rener=Rener,
+                            Dmass=Dmass, Dmom=Dmom, Dener=Dener, meta=json.dumps(meta))
+        print(name, prob.nelem, prob.npoin, "ok")
+
+
+if __name__ == "__main__":
+    main()

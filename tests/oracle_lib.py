@@ -1,0 +1,142 @@
+"""ctypes bindings of the checkers under oracle/ (test infrastructure).
+
+* ``Oracle``: the C restatement of the reference algorithm
+  (oracle/msod_oracle.c, built into oracle/_build/libmsod_oracle.so).
+* ``RefLib``: the reference's own Fortran kernels compiled by oracle/Makefile
+  into oracle/_ref/libsodref_pP.so (only where they have been built).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libmsod_oracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+_vp = ctypes.c_void_p
+_d = ctypes.c_double
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f(shape, dtype=np.float64):
+    return np.zeros(shape, dtype=dtype, order="F")
+
+
+class Oracle:
+    def __init__(self):
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (make -C oracle oracle)")
+        L = ctypes.CDLL(ORACLE_SO, mode=ctypes.RTLD_LOCAL)
+        L.msod_oracle_convec.argtypes = [ctypes.c_int] * 3 + [_vp] * 16 + [ctypes.c_int]
+        L.msod_oracle_diffu.argtypes = [ctypes.c_int] * 3 + [_vp] * 8 + [_d] * 4 + [_vp] * 9 + [ctypes.c_int]
+        self.L = L
+
+    def gll(self, p, exact=False):
+        xi, w = np.zeros(p + 1), np.zeros(p + 1)
+        fn = self.L.msod_oracle_gll_double if exact else self.L.msod_oracle_gll
+        fn(p, _vp(xi.ctypes.data), _vp(w.ctypes.data))
+        return xi, w
+
+    def tables(self, p):
+        q = p + 1
+        nn = q ** 3
+        t = [_f((nn, 3), np.int32), _f((q, q, q), np.int32), _f(nn, np.int32), _f(nn, np.int32),
+             _f(nn, np.int32), _f(nn, np.int32), _f((nn, 3)), _f(nn), _f((nn, nn)),
+             _f((3, nn, nn)), _f((nn, 3, q))]
+        self.L.msod_oracle_tables(p, *[_vp(x.ctypes.data) for x in t])
+        return dict(zip("gmshIJK invAtoIJK AtoIJK AtoI AtoJ AtoK xgp wgp Ngp dNgp dlxigp_ip".split(), t))
+
+    def jacobian(self, p, nelem, npoin, connec, coord, dNgp, wgp):
+        nn = (p + 1) ** 3
+        He, gp = _f((3, 3, nn, nelem)), _f((1, nn, nelem))
+        self.L.msod_oracle_elem_jacobian(p, nelem, npoin, *[_vp(_p(a)) for a in (connec, coord, dNgp, wgp, gp, He)])
+        return He, gp
+
+    def convec(self, prob, nthreads=1):
+        t = prob.tables
+        R = prob.outputs()
+        self.L.msod_oracle_convec(prob.porder, prob.nelem, prob.npoin, *map(_p, (
+            prob.connec, prob.He, prob.gpvol, t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK,
+            prob.u, prob.q, prob.rho, prob.pr, prob.E)), *map(_p, R), nthreads)
+        return R
+
+    def diffu(self, prob, nthreads=1, mu_sgs=None, c_rho=1.0, c_ener=1.0):
+        t = prob.tables
+        R = prob.outputs()
+        self.L.msod_oracle_diffu(prob.porder, prob.nelem, prob.npoin, *map(_p, (
+            prob.connec, prob.He, prob.gpvol, t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK)),
+            prob.Cp, prob.Pr, c_rho, c_ener, *map(_p, (
+                prob.rho, prob.u, prob.Tem, prob.mu_fluid, prob.mu_e,
+                prob.mu_sgs if mu_sgs is None else mu_sgs)), *map(_p, R), nthreads)
+        return R
+
+
+def ref_available(p):
+    return os.path.exists(os.path.join(REF_DIR, f"libsodref_p{p}.so"))
+
+
+class RefLib:
+    """The reference's Fortran kernels (rp=8) for one porder."""
+
+    def __init__(self, p):
+        path = os.path.join(REF_DIR, f"libsodref_p{p}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference library not built: {path}")
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        assert L.sodref_porder() == p and L.sodref_rp() == 8
+        L.sodref_full_convec_ijk.argtypes = [ctypes.c_int] * 2 + [_vp] * 19
+        L.sodref_full_convec_ijk_split.argtypes = [ctypes.c_int] * 2 + [_vp] * 19
+        L.sodref_full_diffusion_ijk.argtypes = [ctypes.c_int] * 2 + [_vp] * 11 + [_d] * 4 + [_vp] * 9
+        L.sodref_elem_jacobian.argtypes = [ctypes.c_int] * 2 + [_vp] * 6
+        L.sodref_create_mesh.argtypes = [ctypes.c_int] * 2 + [_vp] * 3
+        self.L = L
+        self.p = p
+
+    def tables(self):
+        q = self.p + 1
+        nn = q ** 3
+        t = [_f((nn, 3), np.int32), _f((q, q, q), np.int32), _f(nn, np.int32), _f(nn, np.int32),
+             _f(nn, np.int32), _f(nn, np.int32), _f((nn, 3)), _f(nn), _f((nn, nn)),
+             _f((3, nn, nn)), _f((nn, 3, q))]
+        self.L.sodref_tables(*[_vp(x.ctypes.data) for x in t])
+        return dict(zip("gmshIJK invAtoIJK AtoIJK AtoI AtoJ AtoK xgp wgp Ngp dNgp dlxigp_ip".split(), t))
+
+    def jacobian(self, nelem, npoin, connec, coord, dNgp, wgp):
+        nn = (self.p + 1) ** 3
+        He, gp = _f((3, 3, nn, nelem)), _f((1, nn, nelem))
+        self.L.sodref_elem_jacobian(nelem, npoin, *map(_p, (connec, coord, dNgp, wgp, gp, He)))
+        return He, gp
+
+    def _cargs(self, prob):
+        t = prob.tables
+        return (prob.nelem, prob.npoin, _p(prob.connec), _p(t.Ngp), _p(t.dNgp), _p(prob.He),
+                _p(prob.gpvol), _p(t.dlxigp_ip), _p(t.xgp), _p(t.invAtoIJK), _p(t.AtoI),
+                _p(t.AtoJ), _p(t.AtoK), _p(prob.u), _p(prob.q), _p(prob.rho), _p(prob.pr),
+                _p(prob.E))
+
+    def convec(self, prob):
+        """Rmass, Rmom of full_convec_ijk; Rener of full_convec_ijk_split."""
+        Rf = prob.outputs()
+        Rs = prob.outputs()
+        self.L.sodref_full_convec_ijk(*self._cargs(prob), *map(_p, Rf))
+        self.L.sodref_full_convec_ijk_split(*self._cargs(prob), *map(_p, Rs))
+        return Rf[0], Rf[1], Rs[2], Rs
+
+    def diffu(self, prob, c_rho=1.0, c_ener=1.0):
+        t = prob.tables
+        R = prob.outputs()
+        self.L.sodref_full_diffusion_ijk(
+            prob.nelem, prob.npoin, *map(_p, (prob.connec, t.Ngp, t.dNgp, prob.He, prob.gpvol,
+                                              t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK)),
+            prob.Cp, prob.Pr, c_rho, c_ener,
+            *map(_p, (prob.rho, prob.u, prob.Tem, prob.mu_fluid, prob.mu_e, prob.mu_sgs)), *map(_p, R))
+        return R
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(np.ravel(b))
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / (nb if nb > 0 else 1.0))

@@ -1,0 +1,227 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the reference's
+outputs (golden fixtures) and the CPU oracle.  Bar: relative L2 per residual
+component <= 1e-12 for the FP64 kernels (atomic summation order is the only
+expected difference), <= 1e-6 for the mixed-precision kernels against the
+FP64 oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import minisod2d_amd as m
+from oracle_lib import Oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = sorted(f[:-4] for f in os.listdir(GOLD) if f.startswith("case_"))
+KEYS = ("Rmass", "Rmom", "Rener", "Dmass", "Dmom", "Dener")
+TOL64 = 1e-12
+TOL_MIXED = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_residuals(prob, mixed=False):
+    """Both kernels on the GPU; returns (Rmass, Rmom, Rener, Dmass, Dmom, Dener)."""
+    R = prob.outputs()
+    D = prob.outputs()
+    t = prob.tables
+    if mixed:
+        f = prob.fp32()
+        ins = (prob.connec, f["He"], f["gpvol"], f["u"], f["q"], f["rho"], f["E"], f["Tem"],
+               f["mu_fluid"], f["mu_e"], f["mu_sgs"])
+    else:
+        ins = prob.device_inputs()
+    m.copyin(*ins)
+    m.create(*R, *D)
+    try:
+        if mixed:
+            m.full_convec_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"],
+                                    t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, f["u"],
+                                    f["q"], f["rho"], f["E"], prob.gamma, *R)
+            m.full_diffusion_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"],
+                                       t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp,
+                                       prob.Pr, f["rho"], f["u"], f["Tem"], f["mu_fluid"],
+                                       f["mu_e"], f["mu_sgs"], *D)
+        else:
+            m.full_convec_ijk(*prob.convec_args(*R))
+            m.full_diffusion_ijk(*prob.diffu_args(*D))
+        m.update_host(*R, *D)
+    finally:
+        m.delete(*ins, *R, *D)
+    return R + D
+
+
+def load_case(name):
+    import json
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    return {k: z[k] for k in z.files if k != "meta"}, json.loads(str(z["meta"]))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_matches_reference_golden(gpu, name):
+    from golden.make_golden import CASES as BUILDERS, input_digests
+    say(f"[gpu] golden case {name}")
+    gold, meta = load_case(name)
+    prob = BUILDERS[name]()
+    assert input_digests(prob) == meta["inputs"]
+    out = gpu_residuals(prob)
+    for key, got in zip(KEYS, out):
+        err = rel_l2(got, gold[key])
+        assert err <= TOL64, (key, err)
+
+
+@pytest.mark.parametrize("p", range(1, 9))
+def test_fp64_matches_oracle_all_porders(gpu, p):
+    say(f"[gpu] porder {p} jittered 3x2x2 box vs oracle")
+    prob = m.box_problem(3, p, ny=2, nz=2, jitter=0.05, rho_jump=0.1, seeds=(21, 22, 23, 24))
+    o = Oracle()
+    ref = o.convec(prob) + o.diffu(prob)
+    out = gpu_residuals(prob)
+    for key, got, want in zip(KEYS, out, ref):
+        assert rel_l2(got, want) <= TOL64, (key, rel_l2(got, want))
+
+
+def test_fp64_config1_64cubed_p4_jittered(gpu):
+    """64^3 elements, porder 4, jittered interior nodes (~17M nodes)."""
+    say("[gpu] 64^3 p=4 jittered: generating")
+    prob = m.box_problem(64, 4, jitter=0.05, threads=min(16, os.cpu_count() or 1))
+    say("[gpu] 64^3 p=4: GPU kernels")
+    out = gpu_residuals(prob)
+    say("[gpu] 64^3 p=4: oracle")
+    o = Oracle()
+    nt = min(16, os.cpu_count() or 1)
+    ref = o.convec(prob, nt) + o.diffu(prob, nt)
+    for key, got, want in zip(KEYS, out, ref):
+        err = rel_l2(got, want)
+        say(f"    {key}: rel L2 {err:.3e}")
+        assert err <= TOL64, (key, err)
+    # size-independent property: the weak divergence of tau sums to zero
+    for r in range(3):
+        assert abs(out[4][:, r].sum()) <= 1e-12 * np.abs(out[4][:, r]).sum()
+
+
+def rounded_to_fp32(prob):
+    """The same problem with geometry and state rounded to FP32 (widened back)."""
+    import copy
+    q = copy.copy(prob)
+    for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs"):
+        setattr(q, k, np.asfortranarray(getattr(prob, k).astype(np.float32).astype(np.float64)))
+    # the mixed convective kernel recovers p from E, rho, u
+    ke = 0.5 * (q.u ** 2).sum(axis=1)
+    q.pr = (prob.gamma - 1.0) * (q.E - q.rho * ke)
+    return q
+
+
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (7, 3)])
+def test_mixed_precision_error(gpu, p, n):
+    say(f"[gpu] mixed precision p={p} n={n}")
+    prob = m.box_problem(n, p, jitter=0.05)
+    o = Oracle()
+    ref = o.convec(prob, 4) + o.diffu(prob, 4)
+    q32 = rounded_to_fp32(prob)
+    ref32 = o.convec(q32, 4) + o.diffu(q32, 4)
+    out = gpu_residuals(prob, mixed=True)
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, out, ref)}
+    kern = {k: rel_l2(g, w) for k, g, w in zip(KEYS, out, ref32)}
+    say("    vs FP64 oracle on FP64 inputs: " + " ".join(f"{k}={v:.2e}" for k, v in e2e.items()))
+    say("    vs FP64 oracle on the FP32 inputs: " + " ".join(f"{k}={v:.2e}" for k, v in kern.items()))
+    for k in KEYS:
+        assert kern[k] <= 4 * TOL_MIXED and e2e[k] <= 4 * TOL_MIXED, (k, kern[k], e2e[k])
+
+
+def test_repeat_runs_differ_only_by_summation_order(gpu):
+    say("[gpu] determinism")
+    prob = m.box_problem(6, 4, jitter=0.05)
+    a = gpu_residuals(prob)
+    b = gpu_residuals(prob)
+    for x, y in zip(a, b):
+        assert rel_l2(x, y) <= 1e-14
+
+
+def test_async_and_timing(gpu):
+    say("[gpu] async + timing")
+    prob = m.box_problem(4, 4)
+    R = prob.outputs()
+    ins = prob.device_inputs()
+    m.copyin(*ins)
+    m.create(*R)
+    try:
+        m.timing_reset()
+        m.timing_enable(True)
+        m.set_async(True)
+        for _ in range(3):
+            m.full_convec_ijk(*prob.convec_args(*R))
+        m.synchronize()
+        m.set_async(False)
+        ms, n = m.timing_get(m.TIMER_CONVEC)
+        assert n == 3 and ms > 0
+        zms, zn = m.timing_get(m.TIMER_ZERO)
+        assert zn == 3
+        m.update_host(*R)
+        want = Oracle().convec(prob)
+        for g, w in zip(R, want):
+            assert rel_l2(g, w) <= TOL64
+    finally:
+        m.timing_enable(False)
+        m.set_async(False)
+        m.delete(*ins, *R)
+
+
+def test_error_paths(gpu):
+    say("[gpu] error paths")
+    prob = m.box_problem(2, 3)
+    R = prob.outputs()
+    # nothing present yet
+    with pytest.raises(m.MsodError) as e:
+        m.full_convec_ijk(*prob.convec_args(*R))
+    assert e.value.status == m.MSOD_ERR_NOT_PRESENT
+    ins = prob.device_inputs()
+    m.copyin(*ins)
+    m.create(*R)
+    try:
+        # inconsistent 1-D derivative table
+        args = list(prob.convec_args(*R))
+        bad_dl = prob.tables.dlxigp_ip.copy(order="F")
+        bad_dl[5, 1, 2] += 1.0
+        args[7] = bad_dl
+        with pytest.raises(m.MsodError) as e:
+            m.full_convec_ijk(*args)
+        assert e.value.status == m.MSOD_ERR_TABLE
+        # porder without a kernel
+        L = m.hip_lib()
+        st = L.msod_full_convec_ijk(9, prob.nelem, prob.npoin, *[m._ptr(a) for a in prob.convec_args(*R)[2:]])
+        assert st == m.MSOD_ERR_PORDER
+        # connec entry out of range: rejected before any kernel reads through it
+        bad_c = prob.connec.copy(order="F")
+        bad_c[1, 3] = prob.npoin + 5
+        m.copyin(bad_c)
+        try:
+            args = list(prob.convec_args(*R))
+            args[2] = bad_c
+            with pytest.raises(m.MsodError) as e:
+                m.full_convec_ijk(*args)
+            assert e.value.status == m.MSOD_ERR_MESH
+        finally:
+            m.delete(bad_c)
+        # output array too small for npoin
+        small = np.zeros(prob.npoin - 1)
+        m.create(small)
+        try:
+            with pytest.raises(m.MsodError) as e:
+                m.full_convec_ijk(*prob.convec_args(small, R[1], R[2]))
+            assert e.value.status == m.MSOD_ERR_MESH
+        finally:
+            m.delete(small)
+        # the valid call still works after the failures
+        m.full_convec_ijk(*prob.convec_args(*R))
+        m.update_host(*R)
+        for g, w in zip(R, Oracle().convec(prob)):
+            assert rel_l2(g, w) <= TOL64
+    finally:
+        m.delete(*ins, *R)
