@@ -156,6 +156,10 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--assembly", choices=("colored", "atomic"), default="colored",
+                    help="node assembly of the reported run")
+    ap.add_argument("--compare-assembly", action="store_true",
+                    help="also time the other assembly mode (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported here "
@@ -176,6 +180,13 @@ def main():
     m.copyin(*ins)
     m.create(*R, *D)
     log(f"uploaded in {time.time()-t0:.1f}s; timing FP64")
+    mode = m.ASSEMBLY_ATOMIC if args.assembly == "atomic" else m.ASSEMBLY_COLORED
+    other = None
+    if args.compare_assembly:
+        m.set_assembly(m.ASSEMBLY_COLORED if mode == m.ASSEMBLY_ATOMIC else m.ASSEMBLY_ATOMIC)
+        other = timed(prob, R, D, args.steps, args.warmup, mixed=False)
+        log(f"FP64 other assembly {other}")
+    m.set_assembly(mode)
     r64 = timed(prob, R, D, args.steps, args.warmup, mixed=False)
     log(f"FP64 {r64}")
     parity = None
@@ -219,7 +230,13 @@ def main():
                        "zero_fill": r64["zero_ms"]},
         "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,
         "parity_sampled_rel_l2": parity,
+        "assembly": args.assembly,
     }
+    if other is not None:
+        rec["other_assembly"] = {"mode": "atomic" if args.assembly == "colored" else "colored",
+                                 "ms_per_step": other["ms_per_step"],
+                                 "kernels_ms": {"convec": other["convec_ms"], "diffu": other["diffu_ms"],
+                                                "zero_fill": other["zero_ms"]}}
     if rmx is not None:
         bcm, bdm = canonical_bytes(E, N, 4, 4)
         rec["mixed"] = {"value": N / (rmx["ms_per_step"] * 1e-3), "ms_per_step": rmx["ms_per_step"],

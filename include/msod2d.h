@@ -71,6 +71,15 @@ void *msod_data_deviceptr(const void *host);     /* NULL if not present */
 int msod_data_invalidate(const void *ptr);
 
 /* ---- execution control -------------------------------------------------- */
+/* How the element contributions meet at shared nodes (no counterpart in the
+ * reference, which uses `!$acc atomic update`, elem_convec.f90:414-425):
+ *   MSOD_ASSEMBLY_COLORED (default): per-batch node sums in LDS, batches
+ *     coloured so that one launch per colour writes each node without atomics;
+ *     the summation order is fixed, results are bitwise reproducible.
+ *   MSOD_ASSEMBLY_ATOMIC: the same per-batch sums, FP64 atomics into zeroed
+ *     outputs (one launch). */
+enum { MSOD_ASSEMBLY_COLORED = 0, MSOD_ASSEMBLY_ATOMIC = 1 };
+int msod_set_assembly(int mode);
 int msod_set_async(int async);
 int msod_synchronize(void);
 /* HIP-event timing of every kernel launch on the library stream */

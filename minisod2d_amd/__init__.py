@@ -45,7 +45,7 @@ ABI_SYMBOLS = (
     "msod_init", "msod_finalize", "msod_status_string", "msod_last_error",
     "msod_data_copyin", "msod_data_create", "msod_data_update_device",
     "msod_data_update_host", "msod_data_delete", "msod_data_present",
-    "msod_data_deviceptr", "msod_data_invalidate", "msod_set_async",
+    "msod_data_deviceptr", "msod_data_invalidate", "msod_set_assembly", "msod_set_async",
     "msod_synchronize", "msod_timing_enable", "msod_timing_get", "msod_timing_reset",
     "msod_full_convec_ijk", "msod_full_diffusion_ijk", "msod_set_diffusion_coefficients",
     "msod_full_convec_ijk_mixed", "msod_full_diffusion_ijk_mixed",
@@ -169,6 +169,16 @@ def delete(*arrays: np.ndarray) -> None:
 
 def present(a: np.ndarray) -> bool:
     return bool(hip_lib().msod_data_present(a.ctypes.data))
+
+
+ASSEMBLY_COLORED = 0
+ASSEMBLY_ATOMIC = 1
+
+
+def set_assembly(mode: int) -> None:
+    """ASSEMBLY_COLORED (default, atomic-free, bitwise reproducible) or
+    ASSEMBLY_ATOMIC (FP64 atomics into zeroed outputs)."""
+    _check(hip_lib().msod_set_assembly(mode))
 
 
 def set_async(flag: bool) -> None:

@@ -1,16 +1,32 @@
 // Internal interface between the runtime (msod_runtime.hip) and the kernel
-// translation units.  All pointers are device pointers in the kernels'
-// element-major layouts:
+// translation units.  All pointers are device pointers.
 //
-//   conn   [nelem][nn]       uint32: bits 0..30 = 0-based global node,
-//                            bit 31 = node referenced by this element only
-//   geo    [nelem][9][nn]    He(i,k) at component c = i + 3k (SoA per element)
-//   gvol   [nelem][nn]       gpvol
-// where the element-local node index is lexicographic in the reference's
-// 1-D SOD order: l = I + q*(J + q*K), with (I,J,K) = (AtoI,AtoJ,AtoK)-1.
-// tab.D[q*q] is the 1-D derivative D[i*q+n] = l_n'(xi_i) and tab.l2g[nn]
-// maps l -> 0-based reference (Gmsh) node number, used to read arrays kept
-// in the reference layout (mu_e, mu_sgs).
+// Element data are element-major, with the element-local node index l
+// lexicographic in the reference's 1-D SOD order: l = I + q*(J + q*K),
+// (I,J,K) = (AtoI,AtoJ,AtoK) - 1.
+//   geo   [nelem][9][nn]   He(i,k) at component c = i + 3k (SoA per element)
+//   gvol  [nelem][nn]      gpvol
+// tab.D[q*q] is the 1-D derivative D[i*q+n] = l_n'(xi_i); tab.l2g[nn] maps l
+// to the 0-based reference (Gmsh) node number, used for arrays kept in the
+// reference layout (mu_e, mu_sgs).
+//
+// Assembly plan (built once per mesh by the runtime).  Elements are grouped
+// into batches of Shape<P>::BATCH consecutive elements; one workgroup owns a
+// batch, sums the contributions of its elements to every node of the batch in
+// LDS, and writes each of the batch's distinct nodes once:
+//   conn  [nelem][nn]   0-based global node of each element node
+//   lidx  [nelem][nn]   position of that node in its batch's node list
+//   uoff  [nbatch+1]    node list of batch b: unode[uoff[b] .. uoff[b+1])
+//   unode [..]          sorted distinct global nodes of each batch; bit 31 set
+//                       when this batch is the first writer of the node
+//   batches[nb]         the batches processed by one launch
+// Two ways to combine the batches' partial sums at nodes shared between
+// batches (mode):
+//   kColored: batches are coloured so that batches of one colour share no
+//     node; one launch per colour, each node written with a plain store by its
+//     first batch and read-add-stored by later ones.  No atomics, no zero-fill,
+//     and the summation order is fixed (bitwise reproducible).
+//   kAtomic: one launch over all batches, FP64 atomics into zeroed outputs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,25 +35,44 @@
 namespace msod {
 
 constexpr int kMaxP = 8;
+enum AssemblyMode { kColored = 0, kAtomic = 1 };
 
 template <int P>
 struct Shape {
     static constexpr int Q = P + 1;
     static constexpr int NN = Q * Q * Q;
-    // one node per thread; several elements per block while they fit 256
+    // one node per thread; several elements per pass while they fit 256 threads
     static constexpr int EPB = NN <= 128 ? 256 / NN : 1;
     static constexpr int NT = NN <= 128 ? 256 : ((NN + 63) / 64) * 64;
+    // elements per batch: about 512 element nodes, a whole number of passes
+    static constexpr int BATCH_RAW = 512 / NN < 1 ? 1 : (512 / NN > 64 ? 64 : 512 / NN);
+    static constexpr int BATCH = BATCH_RAW < EPB ? EPB : (BATCH_RAW / EPB) * EPB;
+    static constexpr int NPASS = BATCH / EPB;
+    static constexpr int UMAX = BATCH * NN;
 };
+
+// Host-side copy of Shape<P>::BATCH for run-time porder.
+int batch_size(int porder);
 
 struct Tables {
     const double *D;      // [q*q]
     const int32_t *l2g;   // [nn]
 };
 
+struct Plan {
+    const uint32_t *conn;
+    const uint16_t *lidx;
+    const uint32_t *uoff;
+    const uint32_t *unode;
+    const uint32_t *batches;
+    int64_t nb;
+    int mode;
+};
+
 template <typename T>
 struct ConvecArgs {
     int64_t nelem, npoin;
-    const uint32_t *conn;
+    Plan plan;
     const T *geo, *gvol;
     const T *u, *q, *rho, *pr, *E;   // pr == nullptr: recover pressure from E
     double gamma_m1;                 // gamma-1, used when pr == nullptr
@@ -48,7 +83,7 @@ struct ConvecArgs {
 template <typename T>
 struct DiffuArgs {
     int64_t nelem, npoin;
-    const uint32_t *conn;
+    Plan plan;
     const T *geo, *gvol;
     const T *rho, *u, *Tem, *mu_fluid, *mu_e, *mu_sgs;
     double Cp, Pr, c_rho, c_ener;
@@ -66,31 +101,49 @@ hipError_t launch_diffu(int porder, const DiffuArgs<T> &a, hipStream_t s);
 // Zero Rmass (n), Rmom (3n), Rener (n).
 hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64_t n,
                                hipStream_t s);
+// Zero the listed nodes of the three outputs.
+hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
+                             const uint32_t *nodes, int64_t count, hipStream_t s);
 
-// Element-major layouts from the reference layouts (see top of file).
-// *nbad receives the number of connec entries outside 1..npoin (counted in
-// the device int *bad); conn is written only when that number is zero.
-hipError_t launch_build_conn(const int32_t *connec, int64_t nelem, int nn, int64_t npoin,
-                             const int32_t *l2g, uint32_t *conn, uint32_t *mult, int *bad,
-                             int *nbad, hipStream_t s);
+// Element-major geometry from the reference layouts (see top of file).
 template <typename T>
 hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
                             const int32_t *l2g, T *geo, T *gvol, hipStream_t s);
 
-// XCD-aware mapping: blocks are dispatched round-robin over the 8 XCDs of
-// the chip; give each XCD a contiguous range of element groups so that
-// neighbouring elements (which share nodes) meet in the same L2.
-constexpr int kNumXcd = 8;
-__device__ inline int64_t xcd_group(int64_t ngroups)
+// Write phase shared by the kernels: the batch's node sums sR[c][i]
+// (c = Rmass, Rmom x3, Rener) go to the outputs.
+template <int UMAX>
+__device__ __forceinline__ void write_batch(const Plan &pl, uint32_t u0, uint32_t U,
+                                            const double (*sR)[UMAX], double *Rmass,
+                                            double *Rmom, double *Rener, int64_t npoin)
 {
-    const int64_t b = blockIdx.x;
-    const int64_t per = gridDim.x / kNumXcd;
-    const int64_t g = (b % kNumXcd) * per + b / kNumXcd;
-    return g < ngroups ? g : -1;
+    for (uint32_t i = threadIdx.x; i < U; i += blockDim.x) {
+        const uint32_t w = pl.unode[u0 + i];
+        const int64_t g = w & 0x7fffffffu;
+        double *dst[5] = {Rmass + g, Rmom + g, Rmom + npoin + g, Rmom + 2 * npoin + g, Rener + g};
+        if (pl.mode == kAtomic) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) unsafeAtomicAdd(dst[c], sR[c][i]);
+        } else if (w >> 31) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) *dst[c] = sR[c][i];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) *dst[c] += sR[c][i];
+        }
+    }
 }
-inline int64_t xcd_grid(int64_t ngroups)
+
+// Adds one element node's five residual values into the batch sums.
+template <int UMAX>
+__device__ __forceinline__ void add_node(double (*sR)[UMAX], uint32_t li, double r0, double r1,
+                                         double r2, double r3, double r4)
 {
-    return (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
+    __hip_atomic_fetch_add(&sR[0][li], r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&sR[1][li], r1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&sR[2][li], r2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&sR[3][li], r3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&sR[4][li], r4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 }  // namespace msod

@@ -13,6 +13,8 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <atomic>
 #include <vector>
 
 namespace msod {
@@ -33,29 +35,17 @@ __global__ void k_zero_outputs(double *a, double *b, double *c, int64_t n)
     }
 }
 
-__global__ void k_count_nodes(const int32_t *connec, int64_t total, int64_t npoin, uint32_t *mult,
-                              int *bad)
+__global__ void k_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
+                             const uint32_t *nodes, int64_t count)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int32_t v = connec[i];
-        if (v < 1 || (int64_t)v > npoin)
-            atomicAdd(bad, 1);
-        else
-            atomicAdd(mult + (v - 1), 1u);
-    }
-}
-
-__global__ void k_build_conn(const int32_t *connec, int64_t nelem, int nn, const int32_t *l2g,
-                             const uint32_t *mult, uint32_t *conn)
-{
-    const int64_t total = nelem * nn;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int64_t e = i / nn;
-        const int l = (int)(i - e * nn);
-        const uint32_t g = (uint32_t)(connec[e + nelem * l2g[l]] - 1);
-        conn[i] = g | (mult[g] == 1u ? 0x80000000u : 0u);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const int64_t g = nodes[i];
+        Rmass[g] = 0.0;
+        Rener[g] = 0.0;
+        Rmom[g] = 0.0;
+        Rmom[g + npoin] = 0.0;
+        Rmom[g + 2 * npoin] = 0.0;
     }
 }
 
@@ -89,28 +79,28 @@ hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64
     return hipGetLastError();
 }
 
-hipError_t launch_build_conn(const int32_t *connec, int64_t nelem, int nn, int64_t npoin,
-                             const int32_t *l2g, uint32_t *conn, uint32_t *mult, int *bad,
-                             int *nbad, hipStream_t s)
+hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
+                             const uint32_t *nodes, int64_t count, hipStream_t s)
 {
-    hipError_t err = hipMemsetAsync(mult, 0, sizeof(uint32_t) * npoin, s);
-    if (err != hipSuccess) return err;
-    err = hipMemsetAsync(bad, 0, sizeof(int), s);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_count_nodes, dim3(grid_for(nelem * nn, 256)), dim3(256), 0, s, connec,
-                       nelem * nn, npoin, mult, bad);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    int hbad = 0;
-    err = hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s);
-    if (err != hipSuccess) return err;
-    err = hipStreamSynchronize(s);
-    if (err != hipSuccess) return err;
-    *nbad = hbad;
-    if (hbad != 0) return hipSuccess;
-    hipLaunchKernelGGL(k_build_conn, dim3(grid_for(nelem * nn, 256)), dim3(256), 0, s, connec,
-                       nelem, nn, l2g, mult, conn);
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_nodes, dim3(grid_for(count, 256)), dim3(256), 0, s, Rmass, Rmom, Rener,
+                       npoin, nodes, count);
     return hipGetLastError();
+}
+
+int batch_size(int porder)
+{
+    switch (porder) {
+    case 1: return Shape<1>::BATCH;
+    case 2: return Shape<2>::BATCH;
+    case 3: return Shape<3>::BATCH;
+    case 4: return Shape<4>::BATCH;
+    case 5: return Shape<5>::BATCH;
+    case 6: return Shape<6>::BATCH;
+    case 7: return Shape<7>::BATCH;
+    case 8: return Shape<8>::BATCH;
+    default: return 1;
+    }
 }
 
 template <typename T>
@@ -149,13 +139,28 @@ struct TableSet {
     int32_t *dl2g = nullptr;
 };
 
-struct DerivedConn {
-    const void *src;
-    uint64_t ver;
-    int64_t nelem, npoin;
-    int porder;
+// Assembly plan of one mesh (see msod_kernels.h).
+struct PlanEntry {
+    const void *src = nullptr;
+    uint64_t ver = 0;
+    int64_t nelem = 0, npoin = 0;
+    int porder = 0;
     std::vector<int32_t> l2g;
-    uint32_t *conn;
+    int B = 1;
+    int64_t nbatch = 0;
+    uint32_t *conn = nullptr, *uoff = nullptr, *unode = nullptr, *order = nullptr;
+    uint16_t *lidx = nullptr;
+    std::vector<int64_t> color_off;  // batches of colour c: order[color_off[c] .. color_off[c+1])
+    uint32_t *unref = nullptr;       // nodes no element references (zeroed per call)
+    int64_t nunref = 0;
+    void release()
+    {
+        for (void *p : {(void *)conn, (void *)uoff, (void *)unode, (void *)order, (void *)lidx,
+                        (void *)unref})
+            if (p) (void)hipFree(p);
+        conn = uoff = unode = order = unref = nullptr;
+        lidx = nullptr;
+    }
 };
 
 struct DerivedGeo {
@@ -184,9 +189,9 @@ struct State {
     std::map<uintptr_t, Region> present;
     uint64_t next_version = 1;
     std::vector<TableSet *> tables;
-    std::vector<DerivedConn> conns;
+    std::vector<PlanEntry> plans;
     std::vector<DerivedGeo> geos;
-    int *d_bad = nullptr;
+    int assembly = kColored;
     double ms[kNumTimers] = {0};
     int64_t launches[kNumTimers] = {0};
     std::vector<Pending> pending;
@@ -355,40 +360,179 @@ int get_tables(int porder, const int32_t *invAtoIJK, const int32_t *AtoI, const 
 
 constexpr size_t kMaxDerived = 4;
 
-int get_conn(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, const TableSet *ts,
-             const uint32_t **out)
+template <class F>
+void parallel_for(int64_t n, F f)
 {
-    for (auto &c : g.conns)
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = std::min<unsigned>(16, hw);
+    if (n < 4096 || nt == 1) {
+        f(int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (unsigned i = 0; i < nt; ++i) {
+        const int64_t lo = i * chunk, hi = std::min<int64_t>(n, lo + chunk);
+        if (lo < hi) th.emplace_back(f, lo, hi);
+    }
+    for (auto &t : th) t.join();
+}
+
+template <class V>
+int upload(const std::vector<V> &h, V **d, const char *what)
+{
+    if (h.empty()) {
+        *d = nullptr;
+        return MSOD_OK;
+    }
+    HIPCHK(hipMalloc(d, sizeof(V) * h.size()), what);
+    HIPCHK(hipMemcpy(*d, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice), what);
+    return MSOD_OK;
+}
+
+// Builds the assembly plan of a mesh from its connectivity (device array in
+// the reference layout connec(nelem,nnode), 1-based).  Host work, once per
+// mesh: batches of consecutive elements, their sorted distinct nodes, a greedy
+// colouring of the batches (no two batches of one colour share a node), and
+// the first-writer flag of every (batch, node).
+int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet *ts, PlanEntry &P)
+{
+    const int nn = (int)ts->l2g.size();
+    const int64_t E = nelem * nn;
+    if (npoin > 0x7fffffff || E > 0xffffffffLL)
+        return fail(MSOD_ERR_MESH, "mesh too large for 32-bit node indices");
+    P.B = batch_size(ts->porder);
+    P.nbatch = (nelem + P.B - 1) / P.B;
+    std::vector<int32_t> raw(E);
+    HIPCHK(hipMemcpy(raw.data(), dconnec, sizeof(int32_t) * E, hipMemcpyDeviceToHost),
+           "reading connec");
+    std::vector<uint32_t> conn(E);
+    std::atomic<int64_t> bad{0};
+    const int32_t *l2g = ts->l2g.data();
+    parallel_for(nelem, [&](int64_t lo, int64_t hi) {
+        int64_t nb = 0;
+        for (int64_t e = lo; e < hi; ++e)
+            for (int l = 0; l < nn; ++l) {
+                const int32_t v = raw[e + nelem * l2g[l]];
+                if (v < 1 || v > npoin) {
+                    ++nb;
+                    conn[e * nn + l] = 0;
+                } else {
+                    conn[e * nn + l] = (uint32_t)(v - 1);
+                }
+            }
+        bad += nb;
+    });
+    if (bad.load())
+        return fail(MSOD_ERR_MESH, "connec holds %lld node numbers outside 1..npoin (npoin=%lld)",
+                    (long long)bad.load(), (long long)npoin);
+    std::vector<int32_t>().swap(raw);
+
+    // distinct nodes per batch
+    const int B = P.B;
+    const int64_t NB = P.nbatch;
+    std::vector<uint32_t> ucount(NB);
+    parallel_for(NB, [&](int64_t lo, int64_t hi) {
+        std::vector<uint32_t> tmp;
+        for (int64_t b = lo; b < hi; ++b) {
+            const int64_t e0 = b * B, e1 = std::min<int64_t>(nelem, e0 + B);
+            tmp.assign(conn.begin() + e0 * nn, conn.begin() + e1 * nn);
+            std::sort(tmp.begin(), tmp.end());
+            ucount[b] = (uint32_t)(std::unique(tmp.begin(), tmp.end()) - tmp.begin());
+        }
+    });
+    std::vector<uint32_t> uoff(NB + 1);
+    uoff[0] = 0;
+    for (int64_t b = 0; b < NB; ++b) uoff[b + 1] = uoff[b] + ucount[b];
+    std::vector<uint32_t>().swap(ucount);
+    std::vector<uint32_t> unode(uoff[NB]);
+    std::vector<uint16_t> lidx(E);
+    parallel_for(NB, [&](int64_t lo, int64_t hi) {
+        for (int64_t b = lo; b < hi; ++b) {
+            const int64_t e0 = b * B, e1 = std::min<int64_t>(nelem, e0 + B);
+            uint32_t *u = unode.data() + uoff[b];
+            std::copy(conn.begin() + e0 * nn, conn.begin() + e1 * nn, u);
+            std::sort(u, u + (e1 - e0) * nn);
+            const uint32_t U = uoff[b + 1] - uoff[b];
+            std::unique(u, u + (e1 - e0) * nn);
+            for (int64_t k = e0 * nn; k < e1 * nn; ++k)
+                lidx[k] = (uint16_t)(std::lower_bound(u, u + U, conn[k]) - u);
+        }
+    });
+
+    // greedy colouring: colour bit c of mask[node] is set once a batch of
+    // colour c holds the node
+    std::vector<uint32_t> mask(npoin, 0u);
+    std::vector<uint8_t> color(NB);
+    int ncolors = 0;
+    for (int64_t b = 0; b < NB; ++b) {
+        uint32_t forb = 0;
+        for (uint32_t k = uoff[b]; k < uoff[b + 1]; ++k) forb |= mask[unode[k]];
+        if (forb == 0xffffffffu) return fail(MSOD_ERR_MESH, "mesh needs more than 32 batch colours");
+        const int c = __builtin_ctz(~forb);
+        color[b] = (uint8_t)c;
+        ncolors = std::max(ncolors, c + 1);
+        for (uint32_t k = uoff[b]; k < uoff[b + 1]; ++k) mask[unode[k]] |= 1u << c;
+    }
+    // first writer of a node: the batch of the lowest colour holding it
+    parallel_for(NB, [&](int64_t lo, int64_t hi) {
+        for (int64_t b = lo; b < hi; ++b) {
+            const uint32_t lower = (1u << color[b]) - 1u;
+            for (uint32_t k = uoff[b]; k < uoff[b + 1]; ++k)
+                if ((mask[unode[k]] & lower) == 0u) unode[k] |= 0x80000000u;
+        }
+    });
+    std::vector<uint32_t> unref;
+    for (int64_t i = 0; i < npoin; ++i)
+        if (mask[i] == 0u) unref.push_back((uint32_t)i);
+    std::vector<uint32_t>().swap(mask);
+    // batches grouped by colour (stable)
+    P.color_off.assign(ncolors + 1, 0);
+    for (int64_t b = 0; b < NB; ++b) P.color_off[color[b] + 1]++;
+    for (int c = 0; c < ncolors; ++c) P.color_off[c + 1] += P.color_off[c];
+    std::vector<uint32_t> order(NB);
+    {
+        std::vector<int64_t> pos(P.color_off.begin(), P.color_off.end() - 1);
+        for (int64_t b = 0; b < NB; ++b) order[pos[color[b]]++] = (uint32_t)b;
+    }
+    int st;
+    if ((st = upload(conn, &P.conn, "plan upload"))) return st;
+    if ((st = upload(lidx, &P.lidx, "plan upload"))) return st;
+    if ((st = upload(uoff, &P.uoff, "plan upload"))) return st;
+    if ((st = upload(unode, &P.unode, "plan upload"))) return st;
+    if ((st = upload(order, &P.order, "plan upload"))) return st;
+    if ((st = upload(unref, &P.unref, "plan upload"))) return st;
+    P.nunref = (int64_t)unref.size();
+    return MSOD_OK;
+}
+
+int get_plan(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, const TableSet *ts,
+             const PlanEntry **out)
+{
+    for (auto &c : g.plans)
         if (c.src == dconnec && c.ver == ver && c.nelem == nelem && c.npoin == npoin &&
             c.porder == ts->porder && c.l2g == ts->l2g) {
-            *out = c.conn;
+            *out = &c;
             return MSOD_OK;
         }
-    const int nn = (int)ts->l2g.size();
-    DerivedConn c{dconnec, ver, nelem, npoin, ts->porder, ts->l2g, nullptr};
-    uint32_t *mult = nullptr;
-    HIPCHK(hipMalloc(&c.conn, sizeof(uint32_t) * nelem * nn), "hipMalloc element-major connec");
-    HIPCHK(hipMalloc(&mult, sizeof(uint32_t) * npoin), "hipMalloc node counts");
-    int nbad = 0;
-    hipError_t e = launch_build_conn((const int32_t *)dconnec, nelem, nn, npoin, ts->dl2g, c.conn,
-                                     mult, g.d_bad, &nbad, g.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
-    (void)hipFree(mult);
-    if (e == hipSuccess && nbad != 0) {
-        (void)hipFree(c.conn);
-        return fail(MSOD_ERR_MESH, "connec holds node numbers outside 1..npoin (npoin=%lld)",
-                    (long long)npoin);
+    PlanEntry P;
+    P.src = dconnec;
+    P.ver = ver;
+    P.nelem = nelem;
+    P.npoin = npoin;
+    P.porder = ts->porder;
+    P.l2g = ts->l2g;
+    int st = build_plan(dconnec, nelem, npoin, ts, P);
+    if (st) {
+        P.release();
+        return st;
     }
-    if (e != hipSuccess) {
-        (void)hipFree(c.conn);
-        return hipfail(e, "building element-major connec");
+    if (g.plans.size() >= kMaxDerived) {
+        g.plans.front().release();
+        g.plans.erase(g.plans.begin());
     }
-    if (g.conns.size() >= kMaxDerived) {
-        (void)hipFree(g.conns.front().conn);
-        g.conns.erase(g.conns.begin());
-    }
-    g.conns.push_back(c);
-    *out = c.conn;
+    g.plans.push_back(std::move(P));
+    *out = &g.plans.back();
     return MSOD_OK;
 }
 
@@ -429,10 +573,10 @@ int get_geo(const void *dHe, uint64_t vHe, const void *dVol, uint64_t vVol, int6
 void drop_derived_in(uintptr_t lo, uintptr_t hi)
 {
     auto inside = [&](const void *p) { return (uintptr_t)p >= lo && (uintptr_t)p < hi; };
-    for (size_t i = g.conns.size(); i-- > 0;)
-        if (inside(g.conns[i].src)) {
-            (void)hipFree(g.conns[i].conn);
-            g.conns.erase(g.conns.begin() + i);
+    for (size_t i = g.plans.size(); i-- > 0;)
+        if (inside(g.plans[i].src)) {
+            g.plans[i].release();
+            g.plans.erase(g.plans.begin() + i);
         }
     for (size_t i = g.geos.size(); i-- > 0;)
         if (inside(g.geos[i].srcHe) || inside(g.geos[i].srcVol)) {
@@ -509,6 +653,45 @@ int check_sizes(int porder, int nelem, int npoin)
     return MSOD_OK;
 }
 
+// Zero-fill (atomic mode) or unreferenced-node zeroing (coloured mode), then
+// the kernel: one launch per colour, or one launch over all batches.
+template <class A, class L>
+int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *what)
+{
+    Plan pl;
+    pl.conn = P.conn;
+    pl.lidx = P.lidx;
+    pl.uoff = P.uoff;
+    pl.unode = P.unode;
+    pl.mode = g.assembly;
+    const int64_t NP = a.npoin;
+    {
+        TimedLaunch tz(0);
+        if (g.assembly == kAtomic)
+            HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
+        else
+            HIPCHK(launch_zero_nodes(a.Rmass, a.Rmom, a.Rener, NP, P.unref, P.nunref, g.stream),
+                   "zero-fill launch");
+        tz.done();
+    }
+    TimedLaunch tk(timer);
+    if (g.assembly == kAtomic) {
+        pl.batches = P.order;
+        pl.nb = P.nbatch;
+        a.plan = pl;
+        HIPCHK(launch(a), what);
+    } else {
+        for (size_t c = 0; c + 1 < P.color_off.size(); ++c) {
+            pl.batches = P.order + P.color_off[c];
+            pl.nb = P.color_off[c + 1] - P.color_off[c];
+            a.plan = pl;
+            HIPCHK(launch(a), what);
+        }
+    }
+    tk.done();
+    return MSOD_OK;
+}
+
 // Shared front half of the convective calls.
 template <typename T>
 int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const T *He,
@@ -537,15 +720,14 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
-    const uint32_t *conn;
+    const PlanEntry *plan;
     const T *geo, *gvol;
-    if ((st = get_conn(dc, vc, NE, NP, ts, &conn))) return st;
+    if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
     if ((st = get_geo<T>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
     ConvecArgs<T> a;
     a.nelem = NE;
     a.npoin = NP;
-    a.conn = conn;
     a.geo = geo;
     a.gvol = gvol;
     a.u = (const T *)du;
@@ -558,16 +740,10 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     a.Rmom = (double *)dRmo;
     a.Rener = (double *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
-    {
-        TimedLaunch tz(0);
-        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
-        tz.done();
-    }
-    {
-        TimedLaunch tk(timer);
-        HIPCHK(launch_convec<T>(porder, a, g.stream), "convection kernel launch");
-        tk.done();
-    }
+    if ((st = run_assembled(a, *plan, timer,
+                            [&](const ConvecArgs<T> &x) { return launch_convec<T>(porder, x, g.stream); },
+                            "convection kernel launch")))
+        return st;
     return finish_call("convection kernel");
 }
 
@@ -600,15 +776,14 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
-    const uint32_t *conn;
+    const PlanEntry *plan;
     const T *geo, *gvol;
-    if ((st = get_conn(dc, vc, NE, NP, ts, &conn))) return st;
+    if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
     if ((st = get_geo<T>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
     DiffuArgs<T> a;
     a.nelem = NE;
     a.npoin = NP;
-    a.conn = conn;
     a.geo = geo;
     a.gvol = gvol;
     a.rho = (const T *)dr;
@@ -625,16 +800,10 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     a.Rmom = (double *)dRmo;
     a.Rener = (double *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
-    {
-        TimedLaunch tz(0);
-        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
-        tz.done();
-    }
-    {
-        TimedLaunch tk(timer);
-        HIPCHK(launch_diffu<T>(porder, a, g.stream), "diffusion kernel launch");
-        tk.done();
-    }
+    if ((st = run_assembled(a, *plan, timer,
+                            [&](const DiffuArgs<T> &x) { return launch_diffu<T>(porder, x, g.stream); },
+                            "diffusion kernel launch")))
+        return st;
     return finish_call("diffusion kernel");
 }
 
@@ -657,7 +826,6 @@ int msod_init(int device)
     if (device < 0 || device >= n) return fail(MSOD_ERR_ARG, "device %d of %d", device, n);
     HIPCHK(hipSetDevice(device), "hipSetDevice");
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking), "hipStreamCreate");
-    HIPCHK(hipMalloc(&g.d_bad, sizeof(int)), "hipMalloc");
     g.device = device;
     g.init = true;
     return MSOD_OK;
@@ -680,7 +848,6 @@ int msod_finalize(void)
     g.tables.clear();
     for (hipEvent_t e : g.free_events) (void)hipEventDestroy(e);
     g.free_events.clear();
-    (void)hipFree(g.d_bad);
     (void)hipStreamDestroy(g.stream);
     g = State{};
     return MSOD_OK;
@@ -857,6 +1024,15 @@ int msod_timing_reset(void)
         g.ms[i] = 0.0;
         g.launches[i] = 0;
     }
+    return MSOD_OK;
+}
+
+int msod_set_assembly(int mode)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (mode != MSOD_ASSEMBLY_COLORED && mode != MSOD_ASSEMBLY_ATOMIC)
+        return fail(MSOD_ERR_ARG, "assembly mode %d", mode);
+    g.assembly = mode == MSOD_ASSEMBLY_ATOMIC ? kAtomic : kColored;
     return MSOD_OK;
 }
 

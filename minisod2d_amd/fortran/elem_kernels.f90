@@ -1,0 +1,105 @@
+! Drop-in replacements of the reference's kernel modules:
+!   module elem_convec  (kernels/sources/elem_convec.f90:198-211)
+!   module elem_diffu   (kernels/sources/elem_diffu.f90:10-23)
+! Same module and procedure names and the same argument lists, so a caller
+! written for the reference compiles unchanged; the bodies run the gfx950
+! kernels through the C ABI.  Differences a caller can see:
+!   * porder is read from the shape of invAtoIJK (run time) instead of
+!     mod_constants (compile time); the dummies are assumed-shape, and every
+!     array must be contiguous and `present` on the device (msod_copyin /
+!     msod_create of module msod_device, the `enter data` counterparts).
+!   * real kind 8 (the reference's rp is a build option, default 4).
+!   * c_rho and c_ener, module globals of mod_numerical_params in the
+!     reference (mod_numerical_params.f90:26-27), are module variables of
+!     elem_diffu here, with the same defaults.
+!   * Rener of full_convec_ijk is the well-defined energy residual of the
+!     reference's split form (elem_convec.f90:694-697); the fused reference
+!     kernel leaves two of its flux arrays uninitialised (:218,:248-250).
+!   * shapes are checked; a mismatch stops with a message.
+module elem_convec
+
+   use iso_c_binding
+   use msod_device, only: msod_full_convec_ijk, msod_check
+   implicit none
+   private
+   public :: full_convec_ijk
+
+contains
+
+   subroutine full_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                              gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
+      real(8), intent(in), contiguous :: gpvol(:,:,:)
+      integer(4), intent(in), contiguous :: invAtoIJK(:,:,:), gmshAtoI(:), gmshAtoJ(:), gmshAtoK(:)
+      real(8), intent(in), contiguous :: q(:,:), u(:,:), rho(:), pr(:), E(:)
+      real(8), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      integer :: porder
+      porder = size(invAtoIJK, 1) - 1
+      call check_shapes('full_convec_ijk', nelem, npoin, porder, connec, He, gpvol, size(u,1), &
+                        size(Rmass), size(Rmom,1), size(Rener))
+      call msod_check(msod_full_convec_ijk(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, He, &
+                      gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, &
+                      Rmass, Rmom, Rener), 'full_convec_ijk')
+   end subroutine full_convec_ijk
+
+   subroutine check_shapes(who, nelem, npoin, porder, connec, He, gpvol, nu, nr1, nr2, nr3)
+      character(*), intent(in) :: who
+      integer, intent(in) :: nelem, npoin, porder, nu, nr1, nr2, nr3
+      integer(4), intent(in) :: connec(:,:)
+      real(8), intent(in) :: He(:,:,:,:), gpvol(:,:,:)
+      integer :: nn
+      nn = (porder+1)**3
+      if (size(connec,1) /= nelem .or. size(connec,2) /= nn .or. size(He,3) /= nn .or. &
+          size(He,4) < nelem .or. size(gpvol,2) /= nn .or. size(gpvol,3) < nelem .or. &
+          nu < npoin .or. nr1 < npoin .or. nr2 < npoin .or. nr3 < npoin) then
+         write(*,'(a,a,a,i0,a,i0,a,i0)') 'msod: ', who, ': array shapes do not match nelem=', nelem, &
+              ' npoin=', npoin, ' porder=', porder
+         error stop 2
+      end if
+   end subroutine check_shapes
+
+end module elem_convec
+
+module elem_diffu
+
+   use iso_c_binding
+   use msod_device, only: msod_full_diffusion_ijk, msod_set_diffusion_coefficients, msod_check
+   implicit none
+   private
+   public :: full_diffusion_ijk, c_rho, c_ener
+
+   real(8) :: c_rho = 1.0d0
+   real(8) :: c_ener = 1.0d0
+
+contains
+
+   subroutine full_diffusion_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
+                                 Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
+      real(8), intent(in), contiguous :: gpvol(:,:,:)
+      integer(4), intent(in), contiguous :: invAtoIJK(:,:,:), gmshAtoI(:), gmshAtoJ(:), gmshAtoK(:)
+      real(8), intent(in) :: Cp, Pr
+      real(8), intent(in), contiguous :: rho(:), u(:,:), Tem(:), mu_fluid(:), mu_e(:,:), mu_sgs(:,:)
+      real(8), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      integer :: porder, nn
+      porder = size(invAtoIJK, 1) - 1
+      nn = (porder+1)**3
+      if (size(connec,1) /= nelem .or. size(connec,2) /= nn .or. size(He,4) < nelem .or. &
+          size(mu_e,1) /= nelem .or. size(mu_e,2) /= nn .or. size(mu_sgs,1) /= nelem .or. &
+          size(u,1) < npoin .or. size(Rmass) < npoin .or. size(Rmom,1) < npoin) then
+         write(*,'(a,i0,a,i0,a,i0)') 'msod: full_diffusion_ijk: array shapes do not match nelem=', nelem, &
+              ' npoin=', npoin, ' porder=', porder
+         error stop 2
+      end if
+      call msod_check(msod_set_diffusion_coefficients(c_rho, c_ener), 'set_diffusion_coefficients')
+      call msod_check(msod_full_diffusion_ijk(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, He, &
+                      gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, &
+                      mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener), 'full_diffusion_ijk')
+   end subroutine full_diffusion_ijk
+
+end module elem_diffu

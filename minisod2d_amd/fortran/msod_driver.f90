@@ -1,0 +1,305 @@
+! miniSOD timer driver on the gfx950 kernels: counterpart of the reference's
+! program main (app/sources/main.f90).  Builds the element tables, a mesh
+! with its geometry and fields, mirrors them on the GPU, times nruns calls of
+! full_convec_ijk and full_diffusion_ijk (each call synchronous, the
+! reference's timed region, main.f90:444-496), writes timers.dat in the
+! reference's format (main.f90:495,499; the two TET columns are written as 0)
+! and prints the reference's summary (main.f90:504-581) plus node-update rates.
+!
+! Usage: miniSOD [--n N] [--nelem NE] [--porder P] [--nruns R] [--jitter J]
+!                [--rho-jump F] [--mixed] [--assembly colored|atomic] [--quiet]
+!   --n N      conforming N x N x N box of the unit cube (default 8)
+!   --nelem NE the reference's own case instead: NE disjoint elements with
+!              its initial conditions (main.f90:257-413)
+!   --mixed    time the mixed-precision kernels (FP32 state and geometry)
+program miniSOD
+
+   use iso_c_binding
+   use msod_fem, only: msod_hex_tables
+   use msod_mesh, only: msod_box_mesh, msod_box_sizes, msod_disjoint_mesh
+   use msod_geom, only: msod_elem_jacobian
+   use msod_fields, only: msod_flow_fields, msod_viscosity_fields
+   use msod_device
+   use elem_convec, only: full_convec_ijk
+   use elem_diffu, only: full_diffusion_ijk
+   implicit none
+
+   integer :: n = 8, nelem_ref = 0, porder = 4, nruns = 10, nn, q, i, nbad, assembly = 0
+   real(8) :: jitter = 0d0, rho_jump = 0d0
+   logical :: mixed = .false., quiet = .false.
+   integer(8) :: nelem8, npoin8
+   integer :: nelem, npoin
+   integer(c_int), allocatable :: connec(:,:), invAtoIJK(:,:,:), AtoIJK(:), AtoI(:), AtoJ(:), AtoK(:)
+   real(c_double), allocatable :: xgp(:,:), wgp(:), Ngp(:,:), dNgp(:,:,:), dlxigp_ip(:,:,:)
+   real(c_double), allocatable :: coord(:,:), He(:,:,:,:), gpvol(:,:,:)
+   real(c_double), allocatable :: u(:,:), qm(:,:), rho(:), pr(:), E(:), Tem(:), mu_fluid(:), mu_e(:,:), mu_sgs(:,:)
+   real(c_double), allocatable :: Rmass(:), Rmom(:,:), Rener(:), Dmass(:), Dmom(:,:), Dener(:)
+   real(c_float), allocatable :: He4(:,:,:,:), gpvol4(:,:,:), u4(:,:), q4(:,:), rho4(:), E4(:), Tem4(:)
+   real(c_float), allocatable :: muf4(:), mue4(:,:), mus4(:,:)
+   real(8), parameter :: gamma = 1.4d0, Rgas = 287d0
+   real(8) :: Cp, Pra, tc, td, tavg_c, tavg_d, tmax_c, tmax_d, tmin_c, tmin_d
+   integer(8) :: c0, c1, c2, rate
+
+   call parse_args()
+   q = porder + 1
+   nn = q**3
+
+   ! ---- tables ---------------------------------------------------------------
+   allocate(invAtoIJK(q,q,q), AtoIJK(nn), AtoI(nn), AtoJ(nn), AtoK(nn))
+   allocate(xgp(nn,3), wgp(nn), Ngp(nn,nn), dNgp(3,nn,nn), dlxigp_ip(nn,3,q))
+   call msod_hex_tables(porder, invAtoIJK, AtoIJK, AtoI, AtoJ, AtoK, xgp, wgp, Ngp, dNgp, dlxigp_ip)
+
+   ! ---- mesh and geometry ----------------------------------------------------
+   if (nelem_ref > 0) then
+      nelem8 = nelem_ref
+      npoin8 = int(nelem_ref, 8)*nn
+   else
+      call msod_box_sizes(n, n, n, porder, nelem8, npoin8)
+   end if
+   if (npoin8 >= 2_8**31 .or. nelem8*nn >= 2_8**31) stop 'mesh too large for 32-bit indices'
+   nelem = int(nelem8)
+   npoin = int(npoin8)
+   write(*,*) 'Number of elements          = ', nelem
+   write(*,*) 'Element order               = ', porder
+   write(*,*) 'Number of nodes per element = ', nn
+   write(*,*) 'Nodes on mesh               = ', npoin
+   write(*,*) 'Number of runs              = ', nruns
+   write(*,*)
+   allocate(connec(nelem,nn), coord(npoin,3), He(3,3,nn,nelem), gpvol(1,nn,nelem))
+   if (nelem_ref > 0) then
+      call msod_disjoint_mesh(nelem, npoin, nn, xgp, connec, coord)
+   else
+      call msod_box_mesh(n, n, n, porder, jitter, 1004_c_int64_t, connec, coord)
+   end if
+   call msod_elem_jacobian(nelem, npoin, nn, connec, coord, dNgp, wgp, gpvol, He, nbad)
+   if (nbad > 0) stop 'non-positive Jacobian determinant'
+
+   ! ---- fields -----------------------------------------------------------------
+   allocate(u(npoin,3), qm(npoin,3), rho(npoin), pr(npoin), E(npoin), Tem(npoin), mu_fluid(npoin))
+   allocate(mu_e(nelem,nn), mu_sgs(nelem,nn))
+   if (nelem_ref > 0) then
+      call reference_fields()
+      Cp = 1d0
+      Pra = 1d0
+   else
+      call msod_flow_fields(npoin8, coord, gamma, Rgas, 1001_c_int64_t, rho_jump, u, qm, rho, pr, E, Tem, mu_fluid)
+      call msod_viscosity_fields(nelem8, nn, 1002_c_int64_t, 1003_c_int64_t, 1d-3, 1d-4, mu_e, mu_sgs)
+      Cp = 1004.5d0
+      Pra = 0.71d0
+   end if
+   allocate(Rmass(npoin), Rmom(npoin,3), Rener(npoin), Dmass(npoin), Dmom(npoin,3), Dener(npoin))
+
+   ! ---- device data (the reference's `enter data` block) ------------------------
+   call msod_start(0)
+   call msod_set_assembly_mode(assembly)
+   call msod_copyin(connec, c_sizeof(connec))
+   call msod_create(Rmass, c_sizeof(Rmass))
+   call msod_create(Rmom, c_sizeof(Rmom))
+   call msod_create(Rener, c_sizeof(Rener))
+   call msod_create(Dmass, c_sizeof(Dmass))
+   call msod_create(Dmom, c_sizeof(Dmom))
+   call msod_create(Dener, c_sizeof(Dener))
+   if (mixed) then
+      allocate(He4, source=real(He, c_float))
+      allocate(gpvol4, source=real(gpvol, c_float))
+      allocate(u4, source=real(u, c_float))
+      allocate(q4, source=real(qm, c_float))
+      allocate(rho4, source=real(rho, c_float))
+      allocate(E4, source=real(E, c_float))
+      allocate(Tem4, source=real(Tem, c_float))
+      allocate(muf4, source=real(mu_fluid, c_float))
+      allocate(mue4, source=real(mu_e, c_float))
+      allocate(mus4, source=real(mu_sgs, c_float))
+      call msod_copyin(He4, c_sizeof(He4))
+      call msod_copyin(gpvol4, c_sizeof(gpvol4))
+      call msod_copyin(u4, c_sizeof(u4))
+      call msod_copyin(q4, c_sizeof(q4))
+      call msod_copyin(rho4, c_sizeof(rho4))
+      call msod_copyin(E4, c_sizeof(E4))
+      call msod_copyin(Tem4, c_sizeof(Tem4))
+      call msod_copyin(muf4, c_sizeof(muf4))
+      call msod_copyin(mue4, c_sizeof(mue4))
+      call msod_copyin(mus4, c_sizeof(mus4))
+   else
+      call msod_copyin(He, c_sizeof(He))
+      call msod_copyin(gpvol, c_sizeof(gpvol))
+      call msod_copyin(u, c_sizeof(u))
+      call msod_copyin(qm, c_sizeof(qm))
+      call msod_copyin(rho, c_sizeof(rho))
+      call msod_copyin(pr, c_sizeof(pr))
+      call msod_copyin(E, c_sizeof(E))
+      call msod_copyin(Tem, c_sizeof(Tem))
+      call msod_copyin(mu_fluid, c_sizeof(mu_fluid))
+      call msod_copyin(mu_e, c_sizeof(mu_e))
+      call msod_copyin(mu_sgs, c_sizeof(mu_sgs))
+   end if
+
+   ! ---- timing loop (main.f90:427-499) ------------------------------------------
+   open(unit=1, file='timers.dat', status='replace')
+   tavg_c = 0d0; tavg_d = 0d0; tmax_c = 0d0; tmax_d = 0d0; tmin_c = huge(1d0); tmin_d = huge(1d0)
+   call system_clock(count_rate=rate)
+   do i = 1, nruns
+      call system_clock(c0)
+      call convec()
+      call system_clock(c1)
+      call diffu()
+      call system_clock(c2)
+      tc = dble(c1 - c0)/dble(rate)
+      td = dble(c2 - c1)/dble(rate)
+      tavg_c = tavg_c + tc; tmax_c = max(tmax_c, tc); tmin_c = min(tmin_c, tc)
+      tavg_d = tavg_d + td; tmax_d = max(tmax_d, td); tmin_d = min(tmin_d, td)
+      write(1,10) i, tc, 0d0, td, 0d0
+   end do
+   close(1)
+10 format(i3,1x,4(f16.8,1x))
+   tavg_c = tavg_c/dble(nruns)
+   tavg_d = tavg_d/dble(nruns)
+   write(*,*)
+   write(*,*) 'Timings:'
+   write(*,*) '----------------------------------------'
+   write(*,*) 'Avg. convective time     = ', tavg_c
+   write(*,*) 'Avg. diffusive time      = ', tavg_d
+   write(*,*) 'Max. convective time     = ', tmax_c
+   write(*,*) 'Max. diffusive time      = ', tmax_d
+   write(*,*) 'Min. convective time     = ', tmin_c
+   write(*,*) 'Min. diffusive time      = ', tmin_d
+   write(*,*) '----------------------------------------'
+   write(*,*) 'Variation convec.        = ', (tmax_c - tmin_c)/tavg_c
+   write(*,*) 'Variation diffu.         = ', (tmax_d - tmin_d)/tavg_d
+   write(*,*) 'GLL-node updates/s (min) = ', dble(npoin)/(tmin_c + tmin_d)
+   write(*,*) '----------------------------------------'
+
+   ! ---- results (main.f90:539-566) -------------------------------------------------
+   call msod_update_host(Rmass, c_sizeof(Rmass))
+   call msod_update_host(Rmom, c_sizeof(Rmom))
+   call msod_update_host(Rener, c_sizeof(Rener))
+   call msod_update_host(Dmass, c_sizeof(Dmass))
+   call msod_update_host(Dmom, c_sizeof(Dmom))
+   call msod_update_host(Dener, c_sizeof(Dener))
+   if (.not. quiet) then
+      write(*,*)
+      write(*,*) 'Basic results:'
+      write(*,*) '----------------------------------------'
+      write(*,*) '--| Convec'
+      write(*,*) 'Max Rmass     = ', maxval(Rmass), 'Min Rmass     = ', minval(Rmass)
+      write(*,*) 'Max Rmom(:,1) = ', maxval(Rmom(:,1)), 'Min Rmom(:,1) = ', minval(Rmom(:,1))
+      write(*,*) 'Max Rmom(:,2) = ', maxval(Rmom(:,2)), 'Min Rmom(:,2) = ', minval(Rmom(:,2))
+      write(*,*) 'Max Rmom(:,3) = ', maxval(Rmom(:,3)), 'Min Rmom(:,3) = ', minval(Rmom(:,3))
+      write(*,*) 'Max Rener     = ', maxval(Rener), 'Min Rener     = ', minval(Rener)
+      write(*,*) '----------------------------------------'
+      write(*,*) '--| Diffu'
+      write(*,*) 'Max Dmass     = ', maxval(Dmass), 'Min Dmass     = ', minval(Dmass)
+      write(*,*) 'Max Dmom(:,1) = ', maxval(Dmom(:,1)), 'Min Dmom(:,1) = ', minval(Dmom(:,1))
+      write(*,*) 'Max Dmom(:,2) = ', maxval(Dmom(:,2)), 'Min Dmom(:,2) = ', minval(Dmom(:,2))
+      write(*,*) 'Max Dmom(:,3) = ', maxval(Dmom(:,3)), 'Min Dmom(:,3) = ', minval(Dmom(:,3))
+      write(*,*) 'Max Dener     = ', maxval(Dener), 'Min Dener     = ', minval(Dener)
+   end if
+   call write_results()
+   call msod_stop()
+
+contains
+
+   subroutine convec()
+      if (mixed) then
+         call msod_check(msod_full_convec_ijk_mixed(int(porder, c_int), nelem, npoin, connec, He4, gpvol4, &
+                         dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK, u4, q4, rho4, E4, gamma, Rmass, Rmom, Rener), &
+                         'full_convec_ijk_mixed')
+      else
+         call full_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                              AtoI, AtoJ, AtoK, u, qm, rho, pr, E, Rmass, Rmom, Rener)
+      end if
+   end subroutine convec
+
+   subroutine diffu()
+      if (mixed) then
+         call msod_check(msod_full_diffusion_ijk_mixed(int(porder, c_int), nelem, npoin, connec, He4, gpvol4, &
+                         dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK, Cp, Pra, rho4, u4, Tem4, muf4, mue4, mus4, &
+                         Dmass, Dmom, Dener), 'full_diffusion_ijk_mixed')
+      else
+         call full_diffusion_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                                 AtoI, AtoJ, AtoK, Cp, Pra, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
+                                 Dmass, Dmom, Dener)
+      end if
+   end subroutine diffu
+
+   ! The reference driver's initial conditions (main.f90:326-361).
+   subroutine reference_fields()
+      integer :: iel, ip, d
+      real(8) :: fmag, eint
+      u = 1d0; rho = 1d0; pr = 1d0; E = 1d0; Tem = 1d0; mu_fluid = 1d0; mu_e = 1d0; mu_sgs = 1d0
+      do iel = 1, nelem
+         u(connec(iel,1),:) = 10d0
+         u(connec(iel,4),:) = 20d0
+         rho(connec(iel,1)) = 10d0
+         rho(connec(iel,4)) = 20d0
+         pr(connec(iel,1)) = 100d0
+         pr(connec(iel,4)) = 200d0
+      end do
+      do ip = 1, npoin
+         fmag = 0d0
+         do d = 1, 3
+            qm(ip,d) = rho(ip)*u(ip,d)
+            fmag = fmag + qm(ip,d)*u(ip,d)
+         end do
+         eint = pr(ip)/((gamma - 1d0)*rho(ip))
+         E(ip) = rho(ip)*eint + 0.5d0*fmag
+         Tem(ip) = pr(ip)/(rho(ip)*Rgas)
+      end do
+   end subroutine reference_fields
+
+   ! results_convec.dat / results_diffu.dat (main.f90:586-599), small meshes only
+   subroutine write_results()
+      integer :: ip
+      if (npoin > 200000) return
+      open(unit=10, file='results_convec.dat', status='replace')
+      do ip = 1, npoin
+         write(10,*) ip, Rmass(ip), Rmom(ip,1), Rmom(ip,2), Rmom(ip,3), Rener(ip)
+      end do
+      close(10)
+      open(unit=11, file='results_diffu.dat', status='replace')
+      do ip = 1, npoin
+         write(11,*) ip, Dmass(ip), Dmom(ip,1), Dmom(ip,2), Dmom(ip,3), Dener(ip)
+      end do
+      close(11)
+   end subroutine write_results
+
+   subroutine parse_args()
+      integer :: k, nargs
+      character(len=64) :: a, v
+      nargs = command_argument_count()
+      k = 1
+      do while (k <= nargs)
+         call get_command_argument(k, a)
+         select case (trim(a))
+         case ('--mixed')
+            mixed = .true.
+         case ('--quiet')
+            quiet = .true.
+         case ('--n', '--nelem', '--porder', '--nruns', '--jitter', '--rho-jump', '--assembly')
+            if (k == nargs) stop 'missing value'
+            call get_command_argument(k+1, v)
+            select case (trim(a))
+            case ('--n'); read(v,*) n
+            case ('--nelem'); read(v,*) nelem_ref
+            case ('--porder'); read(v,*) porder
+            case ('--nruns'); read(v,*) nruns
+            case ('--jitter'); read(v,*) jitter
+            case ('--rho-jump'); read(v,*) rho_jump
+            case ('--assembly')
+               if (trim(v) == 'atomic') then
+                  assembly = 1
+               else
+                  assembly = 0
+               end if
+            end select
+            k = k + 1
+         case default
+            write(*,*) 'unknown argument ', trim(a)
+            stop 1
+         end select
+         k = k + 1
+      end do
+      if (porder < 1 .or. porder > 8) stop 'porder must be 1..8'
+   end subroutine parse_args
+
+end program miniSOD

@@ -225,3 +225,86 @@ def test_error_paths(gpu):
             assert rel_l2(g, w) <= TOL64
     finally:
         m.delete(*ins, *R)
+
+
+@pytest.mark.parametrize("mode", [m.ASSEMBLY_COLORED, m.ASSEMBLY_ATOMIC])
+def test_assembly_modes(gpu, mode):
+    say(f"[gpu] assembly mode {mode}")
+    m.set_assembly(mode)
+    try:
+        for name in ("case_box8_p3", "case_box2_p7_jit_jump", "case_refdriver_p4"):
+            from golden.make_golden import CASES as BUILDERS
+            gold, _ = load_case(name)
+            out = gpu_residuals(BUILDERS[name]())
+            for key, got in zip(KEYS, out):
+                assert rel_l2(got, gold[key]) <= TOL64, (name, key)
+    finally:
+        m.set_assembly(m.ASSEMBLY_COLORED)
+
+
+def test_colored_assembly_is_bitwise_reproducible(gpu):
+    say("[gpu] bitwise reproducibility of the coloured assembly")
+    prob = m.box_problem(5, 4, jitter=0.05)
+    a = gpu_residuals(prob)
+    b = gpu_residuals(prob)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_shuffled_elements_and_unreferenced_nodes(gpu):
+    """Any element order (batches then share few nodes and need more colours)
+    and nodes that no element references (their residual is zero)."""
+    say("[gpu] shuffled element order + unreferenced nodes")
+    prob = m.box_problem(4, 3, jitter=0.05)
+    rng = np.random.default_rng(5)
+    perm = rng.permutation(prob.nelem)
+    prob.connec = np.asfortranarray(prob.connec[perm])
+    prob.He = np.asfortranarray(prob.He[:, :, :, perm])
+    prob.gpvol = np.asfortranarray(prob.gpvol[:, :, perm])
+    prob.mu_e = np.asfortranarray(prob.mu_e[perm])
+    prob.mu_sgs = np.asfortranarray(prob.mu_sgs[perm])
+    # append 7 unreferenced nodes
+    extra = 7
+    for k in ("u", "q"):
+        setattr(prob, k, np.asfortranarray(np.vstack([getattr(prob, k), np.ones((extra, 3))])))
+    for k in ("rho", "pr", "E", "Tem", "mu_fluid"):
+        setattr(prob, k, np.concatenate([getattr(prob, k), np.ones(extra)]))
+    prob.npoin += extra
+    o = Oracle()
+    ref = o.convec(prob) + o.diffu(prob)
+    for mode in (m.ASSEMBLY_COLORED, m.ASSEMBLY_ATOMIC):
+        m.set_assembly(mode)
+        try:
+            out = gpu_residuals(prob)
+        finally:
+            m.set_assembly(m.ASSEMBLY_COLORED)
+        for key, got, want in zip(KEYS, out, ref):
+            assert rel_l2(got, want) <= TOL64, (mode, key)
+            assert np.all(np.asarray(got)[-extra:] == 0.0)
+
+
+@pytest.mark.parametrize("args,case", [
+    (["--nelem", "6", "--porder", "4"], "case_refdriver_p4"),
+    (["--n", "8", "--porder", "3"], "case_box8_p3"),
+])
+def test_fortran_driver(gpu, tmp_path, args, case):
+    """The Fortran timer driver (drop-in elem_convec/elem_diffu modules over the
+    C ABI) reproduces the reference's residuals and writes timers.dat."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(m.HIP_LIB), "miniSOD")
+    say(f"[gpu] Fortran driver {' '.join(args)}")
+    r = subprocess.run([exe, *args, "--nruns", "3", "--quiet"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    gold, _ = load_case(case)
+    rc = np.loadtxt(tmp_path / "results_convec.dat")
+    rd = np.loadtxt(tmp_path / "results_diffu.dat")
+    assert rel_l2(rc[:, 1], gold["Rmass"]) <= TOL64
+    assert rel_l2(rc[:, 2:5], gold["Rmom"]) <= TOL64
+    assert rel_l2(rc[:, 5], gold["Rener"]) <= TOL64
+    assert rel_l2(rd[:, 1], gold["Dmass"]) <= TOL64
+    assert rel_l2(rd[:, 2:5], gold["Dmom"]) <= TOL64
+    assert rel_l2(rd[:, 5], gold["Dener"]) <= TOL64
+    timers = np.loadtxt(tmp_path / "timers.dat")
+    assert timers.shape == (3, 5) and np.all(timers[:, 1] > 0) and np.all(timers[:, 3] > 0)
+    assert "GLL-node updates/s" in r.stdout
