@@ -85,7 +85,9 @@ int msod_synchronize(void);
 /* HIP-event timing of every kernel launch on the library stream */
 int msod_timing_enable(int enable);
 /* kernel: 0 zero-fill, 1 convec, 2 diffu, 3 convec mixed, 4 diffu mixed,
- *         5 diffu learned closure.  Returns accumulated ms and launch count. */
+ *         5 diffu learned closure, 6 closure training kernels (features,
+ *         predict, sensitivity, gradient, adam, loss).  Returns accumulated ms
+ *         and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
 
@@ -134,6 +136,61 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
                                   double Pr, const float *rho, const float *u, const float *Tem,
                                   const float *mu_fluid, const float *mu_e, const float *mu_sgs,
                                   double *Rmass, double *Rmom, double *Rener);
+
+/* ---- learned subgrid-viscosity closure ----------------------------------
+ * No counterpart in the reference: its driver sets mu_sgs = 1
+ * (main.f90:334) and its LES flags are unused (mod_numerical_params.f90:13-17).
+ * A per-node MLP (12 features -> 32 -> 32 -> 1, SiLU, softplus) predicts
+ * mu_sgs from the velocity gradient, rho, |u| and the element size; weights
+ * are a block of MSOD_CLOSURE_NPARAM floats (layout in
+ * minisod2d_amd/csrc/msod_mlp.h), host or device memory. */
+#define MSOD_CLOSURE_NPARAM 1530
+#define MSOD_CLOSURE_NTRAIN 1505
+#define MSOD_CLOSURE_NFEAT 12
+
+/* full_diffusion_ijk with mu_sgs predicted inside the kernel (bf16 MFMA).
+ * Missing (NULL) or non-finite weights fall back to mu_sgs = 0; *used_model
+ * (if not NULL) tells which was used. */
+int msod_full_diffusion_ijk_closure(int porder, int nelem, int npoin, const int32_t *connec,
+                                    const double *Ngp, const double *dNgp, const double *He,
+                                    const double *gpvol, const double *dlxigp_ip,
+                                    const double *xgp, const int32_t *invAtoIJK,
+                                    const int32_t *gmshAtoI, const int32_t *gmshAtoJ,
+                                    const int32_t *gmshAtoK, double Cp, double Pr,
+                                    const double *rho, const double *u, const double *Tem,
+                                    const double *mu_fluid, const double *mu_e,
+                                    const float *mlp_params, int *used_model, double *Rmass,
+                                    double *Rmom, double *Rener);
+
+/* Training building blocks (all arrays host-present or device pointers).
+ * features: [nelem*nnode][12], element-major, nodes in lexicographic SOD
+ * order l = (I-1) + q((J-1) + q(K-1)); gnu has the same layout. */
+int msod_closure_features(int porder, int nelem, int npoin, const int32_t *connec,
+                          const double *He, const double *gpvol, const double *dlxigp_ip,
+                          const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                          const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const double *rho,
+                          const double *u, float *features);
+/* mu_sgs(nelem,ngaus) (reference layout) from the MLP */
+int msod_closure_predict(int porder, int nelem, const int32_t *invAtoIJK, const float *mlp_params,
+                         const float *features, double *mu_sgs);
+/* gnu(e,a) = d <rerr, (Rmom, Rener)> / d mu_sgs(e,a) of full_diffusion_ijk;
+ * rerr: [4][npoin] (momentum x, y, z, energy) */
+int msod_closure_sensitivity(int porder, int nelem, int npoin, const int32_t *connec,
+                             const double *He, const double *gpvol, const double *dlxigp_ip,
+                             const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                             const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                             const double *rho, const double *u, const double *Tem,
+                             const double *rerr, float *gnu);
+/* grad[MSOD_CLOSURE_NTRAIN] = sum_samples gnu * d mu_sgs / d params */
+int msod_closure_gradient(int64_t nsamples, const float *mlp_params, const float *features,
+                          const float *gnu, float *grad);
+/* one Adam step on the trained part of mlp_params (m, v: moment buffers) */
+int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, int step,
+                      double lr, double beta1, double beta2, double eps);
+/* rerr = 2 w_c (R_c - target_c), *loss = sum w_c (R_c - target_c)^2 over the
+ * momentum and energy residuals; target, rerr: [4][npoin]; weights[4] host */
+int msod_closure_loss(int npoin, const double *Rmom, const double *Rener, const double *target,
+                      const double *weights, double *rerr, double *loss);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,9 @@ ABI_SYMBOLS = (
     "msod_synchronize", "msod_timing_enable", "msod_timing_get", "msod_timing_reset",
     "msod_full_convec_ijk", "msod_full_diffusion_ijk", "msod_set_diffusion_coefficients",
     "msod_full_convec_ijk_mixed", "msod_full_diffusion_ijk_mixed",
+    "msod_full_diffusion_ijk_closure", "msod_closure_features", "msod_closure_predict",
+    "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
+    "msod_closure_loss",
 )
 
 
@@ -95,6 +98,14 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_full_diffusion_ijk.argtypes = [_i, _i, _i] + [_vp] * 11 + [_d, _d] + [_vp] * 9
         L.msod_full_convec_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 12 + [_d] + [_vp] * 3
         L.msod_full_diffusion_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
+        L.msod_full_diffusion_ijk_closure.argtypes = ([_i, _i, _i] + [_vp] * 11 + [_d, _d] +
+                                                      [_vp] * 10)
+        L.msod_closure_features.argtypes = [_i, _i, _i] + [_vp] * 11
+        L.msod_closure_predict.argtypes = [_i, _i] + [_vp] * 4
+        L.msod_closure_sensitivity.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d] + [_vp] * 5
+        L.msod_closure_gradient.argtypes = [ctypes.c_int64] + [_vp] * 4
+        L.msod_closure_adam.argtypes = [_vp] * 4 + [_i, _d, _d, _d, _d]
+        L.msod_closure_loss.argtypes = [_i] + [_vp] * 6
         _hip = L
     return _hip
 
@@ -111,6 +122,9 @@ def host_lib() -> ctypes.CDLL:
         L.msodh_viscosity_fields.argtypes = [ctypes.c_int64, _i, ctypes.c_int64, ctypes.c_int64,
                                              _d, _d, _vp, _vp]
         L.msodh_disjoint_mesh.argtypes = [_i, _i, _i, _vp, _vp, _vp]
+        L.msodh_interp_1d.argtypes = [_i, _i, _vp]
+        L.msodh_prolong_fields.argtypes = [_i] * 6 + [_vp] * 4
+        L.msodh_restrict_residual.argtypes = [_i] * 6 + [_vp] * 4
         _host = L
     return _host
 
@@ -253,6 +267,76 @@ def full_diffusion_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, inv
         *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
         float(Cp), float(Pr),
         *map(_ptr, (rho32, u32, Tem32, mu_fluid32, mu_e32, mu_sgs32, Rmass, Rmom, Rener))))
+
+
+# ---------------------------------------------------------------------------
+# learned subgrid-viscosity closure
+# ---------------------------------------------------------------------------
+CLOSURE_NPARAM = 1530   # weight block (floats); layout in csrc/msod_mlp.h
+CLOSURE_NTRAIN = 1505   # trained part: W1 b1 W2 b2 w3 b3
+CLOSURE_NFEAT = 12
+
+
+def full_diffusion_ijk_closure(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp,
+                               invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem,
+                               mu_fluid, mu_e, mlp_params, Rmass, Rmom, Rener) -> bool:
+    """full_diffusion_ijk with mu_sgs predicted by the MLP inside the kernel.
+    Returns whether the model was used (False: invalid or missing weights,
+    mu_sgs = 0)."""
+    used = ctypes.c_int(0)
+    p = list(map(_ptr, (connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI,
+                        gmshAtoJ, gmshAtoK)))
+    r = list(map(_ptr, (rho, u, Tem, mu_fluid, mu_e, mlp_params)))
+    _check(hip_lib().msod_full_diffusion_ijk_closure(
+        _porder(invAtoIJK), nelem, npoin, *p, float(Cp), float(Pr), *r, ctypes.addressof(used),
+        *map(_ptr, (Rmass, Rmom, Rener))))
+    return bool(used.value)
+
+
+def closure_features(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK,
+                     rho, u, features):
+    """features[nelem*nnode, 12] (element-major, lexicographic nodes)."""
+    _check(hip_lib().msod_closure_features(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He, gpvol, dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK, rho, u,
+                    features))))
+
+
+def closure_predict(nelem, invAtoIJK, mlp_params, features, mu_sgs):
+    """mu_sgs(nelem, nnode) (reference layout) from the MLP."""
+    _check(hip_lib().msod_closure_predict(_porder(invAtoIJK), nelem,
+                                          *map(_ptr, (invAtoIJK, mlp_params, features, mu_sgs))))
+
+
+def closure_sensitivity(nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK,
+                        Cp, rho, u, Tem, rerr, gnu):
+    """gnu[e*nnode + l] = d<rerr, (Rmom, Rener)>/d mu_sgs at element node (e, l)."""
+    _check(hip_lib().msod_closure_sensitivity(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He, gpvol, dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK)), float(Cp),
+        *map(_ptr, (rho, u, Tem, rerr, gnu))))
+
+
+def closure_gradient(nsamples, mlp_params, features, gnu, grad):
+    _check(hip_lib().msod_closure_gradient(int(nsamples),
+                                           *map(_ptr, (mlp_params, features, gnu, grad))))
+
+
+def closure_adam(mlp_params, grad, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    _check(hip_lib().msod_closure_adam(*map(_ptr, (mlp_params, grad, m, v)), int(step),
+                                       float(lr), float(beta1), float(beta2), float(eps)))
+
+
+def closure_loss(npoin, Rmom, Rener, target, weights, rerr) -> float:
+    """sum_c w_c |R_c - target_c|^2 over momentum (3) and energy; writes
+    rerr = 2 w_c (R_c - target_c)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    if w.shape != (4,):
+        raise ValueError("weights: 4 values (momentum x, y, z, energy)")
+    loss = ctypes.c_double()
+    _check(hip_lib().msod_closure_loss(npoin, *map(_ptr, (Rmom, Rener, target, w, rerr)),
+                                       ctypes.addressof(loss)))
+    return loss.value
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,449 @@
+"""Learned subgrid-viscosity closure: training pairs, GPU trainer, weight files.
+
+The reference has no subgrid model: ``mu_sgs`` is an input of
+full_diffusion_ijk (elem_diffu.f90:70,146) that its driver sets to 1
+(main.f90:334).  Here a per-node MLP (12 -> 32 -> 32 -> 1, SiLU, softplus;
+csrc/msod_mlp.h) predicts it from the local velocity gradient, rho, |u| and
+the element size, and is trained on data the method generates itself:
+
+* a fine run: the order-``pf`` (7) residuals Rc_f + Rd_f of the synthetic
+  fields on a box mesh of n^3 affine hexahedra, mu_sgs = 0;
+* the same fields interpolated to the order-``pt`` (3) nodes of the same
+  hexahedra, and the fine residual restricted to the order-``pt`` test
+  functions, P^T (Rc_f + Rd_f) (exact for weak-form residuals, since the
+  order-pt basis lies in the order-pf space);
+* the target of the order-pt diffusion residual,
+      target = P^T (Rc_f + Rd_f) - Rc_t,
+  over momentum (3) and energy; the closure should make Rd_t(mu_sgs) reach it.
+
+Loss: sum_c w_c |Rd_t(mu_sgs)_c - target_c|^2 with w_c = 1/|Rd_t(0)_c - target_c|^2,
+so that the method without the closure (mu_sgs = 0) scores 1 per component.
+The gradient goes through the diffusion operator, which is affine in mu_sgs:
+msod_closure_sensitivity gives d loss / d mu_sgs at every element node and
+msod_closure_gradient back-propagates it through the MLP (bf16 MFMA).  All of
+it runs on the GPU; the host only sees the loss.
+
+Weight files: ``.npz`` (no pickled objects) with the parameter block and the
+Adam state, and a ``.json`` manifest with its sha256, so that training can
+resume and a corrupt or foreign file is refused (the kernels then fall back to
+mu_sgs = 0).
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (CLOSURE_NFEAT, CLOSURE_NPARAM, CLOSURE_NTRAIN, SEED_JITTER, SEED_MUE, SEED_MUSGS,
+               SEED_RHO, Problem, box_problem, closure_adam, closure_features, closure_gradient,
+               closure_loss, closure_predict, closure_sensitivity, copyin, create, delete,
+               full_convec_ijk, full_diffusion_ijk, full_diffusion_ijk_closure, host_lib,
+               update_device, update_host)
+
+NIN, NH = 12, 32
+# parameter block offsets (csrc/msod_mlp.h)
+OW1 = 0
+OB1 = OW1 + NIN * NH
+OW2 = OB1 + NH
+OB2 = OW2 + NH * NH
+OW3 = OB2 + NH
+OB3 = OW3 + NH
+OMEAN = OB3 + 1
+OISTD = OMEAN + NIN
+OSCALE = OISTD + NIN
+assert OMEAN == CLOSURE_NTRAIN and OSCALE + 1 == CLOSURE_NPARAM and NIN == CLOSURE_NFEAT
+
+FORMAT = "msod-closure-v1"
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "closure_p3.npz")
+
+
+class ClosureModelError(RuntimeError):
+    """A weight file that is missing, corrupt or of another format."""
+
+
+# ---------------------------------------------------------------------------
+# order transfer (Fortran host library, fortran/msod_interp.f90)
+# ---------------------------------------------------------------------------
+def interp_1d(pf: int, pt: int) -> np.ndarray:
+    """M[i, n] = l^pf_n(xi^pt_i) in the SOD 1-D node order."""
+    M = np.zeros((pt + 1, pf + 1), order="F")
+    host_lib().msodh_interp_1d(pf, pt, M.ctypes.data)
+    return M
+
+
+def prolong_fields(fine: Problem, coarse: Problem, ff: np.ndarray) -> np.ndarray:
+    """Nodal fields ff(npoin_f, ncomp) of the fine order at the coarse nodes."""
+    ff = np.asfortranarray(ff, dtype=np.float64)
+    nc = ff.shape[1]
+    ft = np.zeros((coarse.npoin, nc), order="F")
+    host_lib().msodh_prolong_fields(fine.porder, coarse.porder, fine.nelem, fine.npoin,
+                                    coarse.npoin, nc, fine.connec.ctypes.data,
+                                    coarse.connec.ctypes.data, ff.ctypes.data, ft.ctypes.data)
+    return ft
+
+
+def restrict_residual(fine: Problem, coarse: Problem, rf: np.ndarray) -> np.ndarray:
+    """Weak-form residuals rf(npoin_f, ncomp) moved to the coarse test functions."""
+    rf = np.asfortranarray(rf, dtype=np.float64)
+    nc = rf.shape[1]
+    rt = np.zeros((coarse.npoin, nc), order="F")
+    host_lib().msodh_restrict_residual(fine.porder, coarse.porder, fine.nelem, fine.npoin,
+                                       coarse.npoin, nc, fine.connec.ctypes.data,
+                                       coarse.connec.ctypes.data, rf.ctypes.data, rt.ctypes.data)
+    return rt
+
+
+def prolong_element_field(fine: Problem, coarse: Problem, mf: np.ndarray) -> np.ndarray:
+    """Element-node field mf(nelem, nnode_f) (reference layout) interpolated
+    inside each element to the coarse element nodes."""
+    tf, tc = fine.tables, coarse.tables
+    qf, qc = tf.porder + 1, tc.porder + 1
+    M = interp_1d(tf.porder, tc.porder)
+    F = np.zeros((fine.nelem, qf, qf, qf))
+    F[:, tf.AtoI - 1, tf.AtoJ - 1, tf.AtoK - 1] = mf
+    T = np.einsum("ai,eijk->eajk", M, F)
+    T = np.einsum("bj,eajk->eabk", M, T)
+    T = np.einsum("ck,eabk->eabc", M, T)
+    return np.asfortranarray(T[:, tc.AtoI - 1, tc.AtoJ - 1, tc.AtoK - 1])
+
+
+# ---------------------------------------------------------------------------
+# training pairs
+# ---------------------------------------------------------------------------
+def gpu_residuals(prob: Problem, mu_sgs: np.ndarray | None = None):
+    """(Rmass, Rmom, Rener) of convection and of diffusion, FP64 on the GPU."""
+    mu = prob.mu_sgs if mu_sgs is None else mu_sgs
+    ins = [prob.connec, prob.He, prob.gpvol, prob.u, prob.q, prob.rho, prob.pr, prob.E,
+           prob.Tem, prob.mu_fluid, prob.mu_e, mu]
+    Rc, Rd = prob.outputs(), prob.outputs()
+    copyin(*ins)
+    create(*Rc, *Rd)
+    try:
+        full_convec_ijk(*prob.convec_args(*Rc))
+        full_diffusion_ijk(*prob.diffu_args(*Rd, mu_sgs=mu))
+        update_host(*Rc, *Rd)
+    finally:
+        delete(*ins, *Rc, *Rd)
+    return Rc, Rd
+
+
+def _mom_ener(Rmom: np.ndarray, Rener: np.ndarray) -> np.ndarray:
+    """[4][npoin] (C order): momentum x, y, z, energy."""
+    return np.ascontiguousarray(np.vstack([Rmom.T, Rener[None, :]]))
+
+
+@dataclass
+class ClosurePair:
+    """An order-pt problem, its diffusion target and the mu_sgs = 0 residual."""
+    coarse: Problem
+    target: np.ndarray     # [4][npoin]
+    r0: np.ndarray         # Rd_t(mu_sgs = 0), [4][npoin]
+    weights: np.ndarray    # [4]
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nsamples(self) -> int:
+        return self.coarse.nelem * self.coarse.tables.nnode
+
+
+def pair_seeds(seed: int):
+    return (SEED_RHO + 17 * seed, SEED_MUE + 17 * seed, SEED_MUSGS + 17 * seed, SEED_JITTER)
+
+
+def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float = 0.0,
+              log=None) -> ClosurePair:
+    """Training pair on n^3 affine hexahedra (see the module docstring)."""
+    t0 = time.time()
+    seeds = pair_seeds(seed)
+    fine = box_problem(n, pf, jitter=0.0, rho_jump=rho_jump, seeds=seeds)
+    fine.mu_sgs[:] = 0.0
+    Rcf, Rdf = gpu_residuals(fine)
+    coarse = box_problem(n, pt, jitter=0.0, rho_jump=rho_jump, seeds=seeds)
+    nodal = np.column_stack([fine.u, fine.q, fine.rho, fine.pr, fine.E, fine.Tem, fine.mu_fluid])
+    ft = prolong_fields(fine, coarse, nodal)
+    coarse.u[:] = ft[:, 0:3]
+    coarse.q[:] = ft[:, 3:6]
+    coarse.rho[:] = ft[:, 6]
+    coarse.pr[:] = ft[:, 7]
+    coarse.E[:] = ft[:, 8]
+    coarse.Tem[:] = ft[:, 9]
+    coarse.mu_fluid[:] = ft[:, 10]
+    coarse.mu_e[:] = prolong_element_field(fine, coarse, fine.mu_e)
+    coarse.mu_sgs[:] = 0.0
+    rt = restrict_residual(fine, coarse, np.column_stack([Rcf[1] + Rdf[1], Rcf[2] + Rdf[2]]))
+    del fine, Rcf, Rdf, nodal, ft
+    Rct, Rd0 = gpu_residuals(coarse)                    # mu_sgs = 0
+    target = _mom_ener(rt[:, 0:3], rt[:, 3]) - _mom_ener(Rct[1], Rct[2])
+    r0 = _mom_ener(Rd0[1], Rd0[2])
+    gap = ((r0 - target) ** 2).sum(axis=1)
+    weights = 1.0 / np.maximum(gap, np.finfo(float).tiny)
+    meta = dict(n=n, seed=seed, seeds=list(seeds), pf=pf, pt=pt, rho_jump=rho_jump,
+                nelem=coarse.nelem, npoin=coarse.npoin, build_s=round(time.time() - t0, 2))
+    if log:
+        log(f"pair n={n} seed={seed}: {coarse.nelem * coarse.tables.nnode} samples, "
+            f"{meta['build_s']} s")
+    return ClosurePair(coarse, target, r0, weights, meta)
+
+
+# ---------------------------------------------------------------------------
+# parameters and weight files
+# ---------------------------------------------------------------------------
+def init_params(features: np.ndarray, scale: float, b3: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Parameter block: standardisation from the training features, small
+    random hidden layers and an output layer near the constant mu_sgs =
+    scale * softplus(b3)."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(CLOSURE_NPARAM, dtype=np.float32)
+    P[OW1:OB1] = rng.normal(0.0, 1.0 / np.sqrt(NIN), NIN * NH)
+    P[OW2:OB2] = rng.normal(0.0, 1.0 / np.sqrt(NH), NH * NH)
+    P[OW3:OB3] = rng.normal(0.0, 0.1 / np.sqrt(NH), NH)
+    P[OB3] = b3
+    f = features.astype(np.float64)
+    mean = f.mean(axis=0)
+    std = f.std(axis=0)
+    P[OMEAN:OISTD] = mean
+    P[OISTD:OSCALE] = np.where(std > 1e-12 * np.maximum(1.0, np.abs(mean)), 1.0 / np.maximum(std, 1e-300), 1.0)
+    P[OSCALE] = scale
+    return P
+
+
+def _sha256(path: str) -> str:
+    h = hashlib.sha256()
+    with open(path, "rb") as fh:
+        for chunk in iter(lambda: fh.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def save_weights(path: str, params: np.ndarray, m: np.ndarray | None = None,
+                 v: np.ndarray | None = None, step: int = 0, meta: dict | None = None) -> str:
+    """Writes path (.npz) and path + '.json' (format, sha256, meta)."""
+    params = np.asarray(params, dtype=np.float32)
+    if params.shape != (CLOSURE_NPARAM,):
+        raise ValueError(f"params: {CLOSURE_NPARAM} floats expected")
+    z = np.zeros(CLOSURE_NTRAIN, np.float32)
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "wb") as fh:
+        np.savez(fh, params=params, adam_m=z if m is None else np.asarray(m, np.float32),
+                 adam_v=z if v is None else np.asarray(v, np.float32), step=np.int64(step))
+    man = dict(format=FORMAT, file=os.path.basename(path), sha256=_sha256(path),
+               nparam=CLOSURE_NPARAM, ntrain=CLOSURE_NTRAIN, step=int(step), meta=meta or {})
+    with open(path + ".json", "w") as fh:
+        json.dump(man, fh, indent=1, sort_keys=True)
+        fh.write("\n")
+    return man["sha256"]
+
+
+def load_weights(path: str = DEFAULT_WEIGHTS) -> dict:
+    """Parameter block, Adam state and manifest of a weight file; raises
+    ClosureModelError if the manifest is missing, the checksum or format
+    differs, or the arrays are malformed."""
+    try:
+        with open(path + ".json") as fh:
+            man = json.load(fh)
+    except (OSError, ValueError) as e:
+        raise ClosureModelError(f"{path}.json: {e}") from None
+    if man.get("format") != FORMAT or man.get("nparam") != CLOSURE_NPARAM:
+        raise ClosureModelError(f"{path}: not a {FORMAT} weight file")
+    try:
+        sha = _sha256(path)
+    except OSError as e:
+        raise ClosureModelError(str(e)) from None
+    if sha != man.get("sha256"):
+        raise ClosureModelError(f"{path}: sha256 {sha} differs from the manifest")
+    try:
+        with np.load(path, allow_pickle=False) as z:
+            out = dict(params=z["params"].astype(np.float32), adam_m=z["adam_m"].astype(np.float32),
+                       adam_v=z["adam_v"].astype(np.float32), step=int(z["step"]), manifest=man)
+    except (OSError, ValueError, KeyError) as e:
+        raise ClosureModelError(f"{path}: {e}") from None
+    if out["params"].shape != (CLOSURE_NPARAM,) or out["adam_m"].shape != (CLOSURE_NTRAIN,):
+        raise ClosureModelError(f"{path}: wrong array shapes")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# GPU trainer
+# ---------------------------------------------------------------------------
+class _Device:
+    """Device-resident arrays of one pair (present table of the C ABI)."""
+
+    def __init__(self, pair: ClosurePair):
+        p = pair.coarse
+        t = p.tables
+        self.pair = pair
+        self.nn = t.nnode
+        self.E = p.nelem * self.nn
+        self.features = np.zeros((self.E, NIN), dtype=np.float32)
+        self.mu = np.zeros((p.nelem, self.nn), order="F")
+        self.gnu = np.zeros(self.E, dtype=np.float32)
+        self.rerr = np.zeros((4, p.npoin))
+        self.R = p.outputs()
+        self.ins = [p.connec, p.He, p.gpvol, p.u, p.rho, p.Tem, p.mu_fluid, p.mu_e, pair.target]
+        copyin(*self.ins)
+        create(self.features, self.mu, self.gnu, self.rerr, *self.R)
+        closure_features(p.nelem, p.npoin, p.connec, p.He, p.gpvol, t.dlxigp_ip, t.invAtoIJK,
+                         t.AtoI, t.AtoJ, t.AtoK, p.rho, p.u, self.features)
+        update_host(self.features)
+
+    def close(self):
+        delete(*self.ins, self.features, self.mu, self.gnu, self.rerr, *self.R)
+
+    def loss(self, params, with_grad: bool, grad_acc=None, grad_tmp=None) -> float:
+        """Loss of the pair; with_grad: adds d loss / d params into grad_acc."""
+        p = self.pair.coarse
+        t = p.tables
+        closure_predict(p.nelem, t.invAtoIJK, params, self.features, self.mu)
+        full_diffusion_ijk(*p.diffu_args(*self.R, mu_sgs=self.mu))
+        L = closure_loss(p.npoin, self.R[1], self.R[2], self.pair.target, self.pair.weights,
+                         self.rerr)
+        if with_grad:
+            closure_sensitivity(p.nelem, p.npoin, p.connec, p.He, p.gpvol, t.dlxigp_ip,
+                                t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, p.Cp, p.rho, p.u, p.Tem,
+                                self.rerr, self.gnu)
+            closure_gradient(self.E, params, self.features, self.gnu, grad_tmp)
+        return L
+
+    def constant_optimum(self) -> float:
+        """Least-squares constant mu_sgs of the pair (the loss is quadratic in
+        a constant mu_sgs since the residual is affine in it)."""
+        p = self.pair.coarse
+        one = np.ones((p.nelem, self.nn), order="F")
+        copyin(one)
+        try:
+            full_diffusion_ijk(*p.diffu_args(*self.R, mu_sgs=one))
+            update_host(*self.R)
+        finally:
+            delete(one)
+        L1 = _mom_ener(self.R[1], self.R[2]) - self.pair.r0
+        w = self.pair.weights[:, None]
+        num = float((w * (self.pair.target - self.pair.r0) * L1).sum())
+        den = float((w * L1 * L1).sum())
+        return num / den if den > 0 else 0.0
+
+
+def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: int = 0,
+          init: dict | None = None, log=None) -> dict:
+    """Adam on the sum of the pairs' losses, full batch, all on the GPU.
+    Returns params, Adam state and the loss history."""
+    devs = [_Device(p) for p in pairs]
+    try:
+        if init is not None:
+            params = init["params"].copy()
+            m, v, step0 = init["adam_m"].copy(), init["adam_v"].copy(), init["step"]
+        else:
+            feats = np.concatenate([d.features for d in devs])
+            cstar = [d.constant_optimum() for d in devs]
+            c = float(np.mean(cstar))
+            # start near the best constant viscosity when one helps, near 0 otherwise
+            scale = abs(c) / np.log(2.0) if c > 0 else max(abs(c), 1e-8)
+            params = init_params(feats, scale=scale, b3=0.0 if c > 0 else -6.0, seed=seed)
+            m = np.zeros(CLOSURE_NTRAIN, np.float32)
+            v = np.zeros(CLOSURE_NTRAIN, np.float32)
+            step0 = 0
+            if log:
+                log(f"constant optimum mu_sgs per pair: {cstar}; scale {scale:.3e}")
+        grad = np.zeros(CLOSURE_NTRAIN, np.float32)
+        gtmp = np.zeros(CLOSURE_NTRAIN, np.float32)
+        copyin(params, m, v, grad, gtmp)
+        hist = []
+        t0 = time.time()
+        try:
+            for it in range(1, steps + 1):
+                L = 0.0
+                if len(devs) == 1:
+                    L = devs[0].loss(params, True, grad_tmp=grad)
+                else:
+                    acc = np.zeros(CLOSURE_NTRAIN, np.float64)
+                    for d in devs:
+                        L += d.loss(params, True, grad_tmp=gtmp)
+                        update_host(gtmp)
+                        acc += gtmp
+                    grad[:] = acc.astype(np.float32)
+                    update_device(grad)
+                closure_adam(params, grad, m, v, step0 + it, lr)
+                hist.append(L)
+                if log and (it == 1 or it % 50 == 0 or it == steps):
+                    log(f"step {step0 + it}: loss {L:.6f} ({time.time() - t0:.1f} s)")
+            update_host(params, m, v)
+        finally:
+            delete(params, m, v, grad, gtmp)
+        return dict(params=params, adam_m=m, adam_v=v, step=step0 + steps, history=hist)
+    finally:
+        for d in devs:
+            d.close()
+
+
+def evaluate(pair: ClosurePair, params: np.ndarray | None) -> dict:
+    """Loss of the pair with the model (or mu_sgs = 0 for params None), per
+    component relative to mu_sgs = 0, through the fused closure kernel."""
+    p = pair.coarse
+    t = p.tables
+    R = p.outputs()
+    ins = [p.connec, p.He, p.gpvol, p.rho, p.u, p.Tem, p.mu_fluid, p.mu_e]
+    copyin(*ins)
+    create(*R)
+    try:
+        used = full_diffusion_ijk_closure(p.nelem, p.npoin, p.connec, t.Ngp, t.dNgp, p.He, p.gpvol,
+                                          t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK,
+                                          p.Cp, p.Pr, p.rho, p.u, p.Tem, p.mu_fluid, p.mu_e,
+                                          params, *R)
+        update_host(*R)
+    finally:
+        delete(*ins, *R)
+    r = _mom_ener(R[1], R[2])
+    per = (pair.weights * ((r - pair.target) ** 2).sum(axis=1))
+    return dict(used_model=used, loss=float(per.sum()), per_component=per.tolist(),
+                gain=float(1.0 - per.sum() / 4.0))
+
+
+# ---------------------------------------------------------------------------
+# command line: python -m minisod2d_amd.closure train|evaluate ...
+# ---------------------------------------------------------------------------
+def _main(argv=None) -> int:
+    from . import init as dev_init
+    ap = argparse.ArgumentParser(prog="python -m minisod2d_amd.closure")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    tr = sub.add_parser("train")
+    tr.add_argument("--sizes", type=int, nargs="+", default=[24, 32, 40])
+    tr.add_argument("--seed", type=int, default=0)
+    tr.add_argument("--steps", type=int, default=600)
+    tr.add_argument("--lr", type=float, default=3e-3)
+    tr.add_argument("--out", default=DEFAULT_WEIGHTS)
+    tr.add_argument("--resume", action="store_true")
+    ev = sub.add_parser("evaluate")
+    ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ev.add_argument("--sizes", type=int, nargs="+", default=[28])
+    ev.add_argument("--seed", type=int, default=7)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    dev_init(0)
+    if a.cmd == "train":
+        pairs = [make_pair(n, seed=a.seed + i, log=log) for i, n in enumerate(a.sizes)]
+        init = load_weights(a.out) if a.resume else None
+        res = train(pairs, steps=a.steps, lr=a.lr, seed=a.seed, init=init, log=log)
+        evals = [evaluate(p, res["params"]) for p in pairs]
+        meta = dict(sizes=a.sizes, seed=a.seed, steps=res["step"], lr=a.lr,
+                    samples=int(sum(p.nsamples for p in pairs)),
+                    loss_first=res["history"][0], loss_last=res["history"][-1],
+                    train_gain=[e["gain"] for e in evals])
+        sha = save_weights(a.out, res["params"], res["adam_m"], res["adam_v"], res["step"], meta)
+        print(json.dumps(dict(out=a.out, sha256=sha, **meta)), flush=True)
+    else:
+        w = load_weights(a.weights)
+        for i, n in enumerate(a.sizes):
+            pair = make_pair(n, seed=a.seed + i, log=log)
+            print(json.dumps(dict(n=n, seed=a.seed + i, model=evaluate(pair, w["params"]),
+                                  zero=evaluate(pair, None))), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main())

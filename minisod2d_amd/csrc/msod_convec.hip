@@ -19,7 +19,8 @@
 //     terms through U_d(a), Q_d(a) (sum_i q_i dE/dx_i = sum_d Q_d(a) D_d[E]).
 //     The result equals the reference's algebra exactly in real arithmetic;
 //     rounding differs at the 1e-16 level.
-//  3. residual x gpvol, added into the batch's per-node sums in LDS.
+//  3. residual x gpvol, added into the batch's per-node sums in LDS in a
+//     fixed order (add_pass).
 // After the passes the batch's distinct nodes are written once each
 // (write_batch: plain store / read-add-store per colour, or FP64 atomics).
 //
@@ -91,8 +92,8 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_convec(ConvecArgs<T> a)
             sF[le][9][l] = qq[2];
         }
         __syncthreads();
-        if (!active) continue;
-
+        double v[5] = {0, 0, 0, 0, 0};
+        if (active) {
         const T *geo = a.geo + e * 9 * NN + l;
         // divergence-type accumulators (coefficient independent of d)
         T aQ = 0, aU = 0, aRHU = 0, aHU = 0, aQU[3] = {0, 0, 0}, aUU[3] = {0, 0, 0};
@@ -150,10 +151,14 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_convec(ConvecArgs<T> a)
         for (int i = 0; i < 3; ++i)
             Rmo[i] = (T)0.25 * (aQU[i] + uu[i] * aQ + qq[i] * aU + r * aUU[i] + uu[i] * sRho +
                                 qGu[i] + uGq[i]) + gP[i];
-        add_node<UMAX>(sR, pl.lidx[e * NN + l], (double)(gv * Rm), (double)(gv * Rmo[0]),
-                       (double)(gv * Rmo[1]), (double)(gv * Rmo[2]), (double)(gv * Re));
+        v[0] = (double)(gv * Rm);
+        v[1] = (double)(gv * Rmo[0]);
+        v[2] = (double)(gv * Rmo[1]);
+        v[3] = (double)(gv * Rmo[2]);
+        v[4] = (double)(gv * Re);
+        }
+        add_pass<UMAX, EPB>(sR, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
     }
-    __syncthreads();
     write_batch<UMAX>(pl, u0, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 

@@ -18,23 +18,36 @@
 //    Rmom_r += W[tau_r], with nu_e and kappa_e of the target node (the
 //    reference multiplies kappa_e inside the sum; same value, other rounding).
 // Each element node's five values are added into the batch's per-node sums
-// in LDS; the batch's distinct nodes are then written once (write_batch).
+// in LDS in a fixed order (add_pass); the batch's distinct nodes are then
+// written once (write_batch).
 //
 // T = double: FP64 form; T = float: mixed form (FP32 sweeps, FP64 node sums
-// and assembly).
+// and assembly).  CLOSURE (FP64 only): mu_sgs is not read but predicted per
+// element node by the learned closure (msod_mlp.h) from the pass-1 velocity
+// gradient, evaluated with bf16 MFMA inside the kernel before the stress is
+// formed; its LDS scratch overlays the field buffers between the passes.
+#include <type_traits>
+
 #include "msod_kernels.h"
+#include "msod_mlp.h"
+#include "msod_closure.h"
 
 namespace msod {
 
-template <int P, typename T>
-__global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a)
+template <int P, typename T, bool CLOSURE>
+__global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a, const float *mlp)
 {
     using S = Shape<P>;
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
+    constexpr size_t kSA = sizeof(T) * EPB * 5 * NN, kSG = sizeof(T) * EPB * 6 * NN;
+    constexpr size_t kScr = CLOSURE ? (size_t)(S::NT / 64) * 64 * kMlpH * sizeof(short) : 0;
+    constexpr size_t kBuf = kSA + kSG > kScr ? kSA + kSG : kScr;
     __shared__ T sD[Q * Q];
-    __shared__ T sA[EPB][5][NN];   // rho, T, u0, u1, u2
-    __shared__ T sG[EPB][6][NN];   // contravariant fluxes of the current direction
+    __shared__ __attribute__((aligned(16))) unsigned char sbuf[kBuf];
     __shared__ double sR[5][UMAX];
+    __shared__ typename std::conditional<CLOSURE, MlpShared, char>::type smlp;
+    auto sA = reinterpret_cast<T (*)[5][NN]>(sbuf);        // rho, T, u0, u1, u2
+    auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf + kSA);  // contravariant fluxes of one direction
 
     const Plan &pl = a.plan;
     const int64_t b = pl.batches[blockIdx.x];
@@ -44,6 +57,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a)
     for (uint32_t i = t; i < U; i += blockDim.x)
 #pragma unroll
         for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
+    if constexpr (CLOSURE) mlp_stage(smlp, mlp);
 
     const int le = t / NN;
     const int l = t - le * NN;
@@ -62,7 +76,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a)
             muf = a.mu_fluid[g];
             const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
             mue = a.mu_e[ge];
-            mus = a.mu_sgs[ge];
+            if constexpr (!CLOSURE) mus = a.mu_sgs ? a.mu_sgs[ge] : (T)0;  // NULL: closure fallback
             sA[le][0][l] = r;
             sA[le][1][l] = a.Tem[g];
             for (int d = 0; d < 3; ++d) {
@@ -74,14 +88,14 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a)
 
         // ---- pass 1: gradients, stress ------------------------------------
         T h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // h[i][d] = He(i,d)
-        T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3], tauU[3] = {0, 0, 0}, gv = 0;
+        T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        T tauU[3] = {0, 0, 0}, gv = 0, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (active) {
             const T *geo = a.geo + e * 9 * NN + l;
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
                 for (int i = 0; i < 3; ++i) h[i][d] = geo[(i + 3 * d) * NN];
-            T G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int Id = d == 0 ? I : (d == 1 ? J : K);
@@ -108,6 +122,19 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a)
                     G[2][i] += h[i][d] * aU2;
                 }
             }
+        }
+        if constexpr (CLOSURE) {
+            __syncthreads();  // every sweep over sA done: the MLP scratch overlays it
+            float f[kMlpIn];
+            if (active)
+                mlp_features<T>(G, r, uu, h, f);
+            else
+#pragma unroll
+                for (int k = 0; k < kMlpIn; ++k) f[k] = smlp.mean[k];
+            const float z3 = mlp_forward_z3(smlp, f, reinterpret_cast<short *>(sbuf) + (t >> 6) * 64 * kMlpH);
+            mus = active ? (T)(smlp.scale * softplusf_(z3)) : (T)0;
+        }
+        if (active) {
             const T divU = G[0][0] + G[1][1] + G[2][2];
             const T mu_f = muf + r * mus;
             const T mu_t = mu_f + mue;
@@ -154,21 +181,26 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a)
             }
             __syncthreads();  // sweep done before sG is overwritten
         }
-        if (!active) continue;
-        const T nu_e = (T)a.c_rho * mue / r;
-        const T kappa = muf * Cp / Pr + (T)a.c_ener * mue * Cp / Pr + Cp * r * mus / (T)0.9;
-        add_node<UMAX>(sR, pl.lidx[e * NN + l], (double)(nu_e * W[0]), (double)W[3], (double)W[4],
-                       (double)W[5], (double)(W[1] + kappa * W[2]));
+        double v[5] = {0, 0, 0, 0, 0};
+        if (active) {
+            const T nu_e = (T)a.c_rho * mue / r;
+            const T kappa = muf * Cp / Pr + (T)a.c_ener * mue * Cp / Pr + Cp * r * mus / (T)0.9;
+            v[0] = (double)(nu_e * W[0]);
+            v[1] = (double)W[3];
+            v[2] = (double)W[4];
+            v[3] = (double)W[5];
+            v[4] = (double)(W[1] + kappa * W[2]);
+        }
+        add_pass<UMAX, EPB>(sR, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
     }
-    __syncthreads();
     write_batch<UMAX>(pl, u0, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 
-template <int P, typename T>
-static hipError_t launch_p(const DiffuArgs<T> &a, hipStream_t s)
+template <int P, typename T, bool CLOSURE = false>
+static hipError_t launch_p(const DiffuArgs<T> &a, hipStream_t s, const float *mlp = nullptr)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_diffu<P, T>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s, a);
+    hipLaunchKernelGGL((k_diffu<P, T, CLOSURE>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s, a, mlp);
     return hipGetLastError();
 }
 
@@ -189,6 +221,21 @@ hipError_t launch_diffu(int porder, const DiffuArgs<T> &a, hipStream_t s)
 }
 
 template hipError_t launch_diffu<double>(int, const DiffuArgs<double> &, hipStream_t);
+
+hipError_t launch_diffu_closure(int porder, const DiffuArgs<double> &a, const float *mlp, hipStream_t s)
+{
+    switch (porder) {
+    case 1: return launch_p<1, double, true>(a, s, mlp);
+    case 2: return launch_p<2, double, true>(a, s, mlp);
+    case 3: return launch_p<3, double, true>(a, s, mlp);
+    case 4: return launch_p<4, double, true>(a, s, mlp);
+    case 5: return launch_p<5, double, true>(a, s, mlp);
+    case 6: return launch_p<6, double, true>(a, s, mlp);
+    case 7: return launch_p<7, double, true>(a, s, mlp);
+    case 8: return launch_p<8, double, true>(a, s, mlp);
+    default: return hipErrorInvalidValue;
+    }
+}
 template hipError_t launch_diffu<float>(int, const DiffuArgs<float> &, hipStream_t);
 
 }  // namespace msod

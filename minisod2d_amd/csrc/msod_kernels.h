@@ -20,8 +20,9 @@
 //   unode [..]          sorted distinct global nodes of each batch; bit 31 set
 //                       when this batch is the first writer of the node
 //   batches[nb]         the batches processed by one launch
-// Two ways to combine the batches' partial sums at nodes shared between
-// batches (mode):
+// Within a batch the element-node values are summed in a fixed order
+// (add_pass).  Two ways to combine the batches' partial sums at nodes shared
+// between batches (mode):
 //   kColored: batches are coloured so that batches of one colour share no
 //     node; one launch per colour, each node written with a plain store by its
 //     first batch and read-add-stored by later ones.  No atomics, no zero-fill,
@@ -134,16 +135,23 @@ __device__ __forceinline__ void write_batch(const Plan &pl, uint32_t u0, uint32_
     }
 }
 
-// Adds one element node's five residual values into the batch sums.
-template <int UMAX>
-__device__ __forceinline__ void add_node(double (*sR)[UMAX], uint32_t li, double r0, double r1,
-                                         double r2, double r3, double r4)
+// Adds the five residual values of every element node of a pass into the
+// batch's node sums, one element of the pass after the other: the nodes of
+// one element are distinct, so no two threads of a round touch the same sum,
+// and the summation order (pass, then element, within the batch) is fixed.
+// Every thread of the block calls it (uniform control flow, barriers).
+template <int UMAX, int EPB>
+__device__ __forceinline__ void add_pass(double (*sR)[UMAX], int le, bool active, uint32_t li,
+                                         const double (&v)[5])
 {
-    __hip_atomic_fetch_add(&sR[0][li], r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&sR[1][li], r1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&sR[2][li], r2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&sR[3][li], r3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&sR[4][li], r4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll 1
+    for (int k = 0; k < EPB; ++k) {
+        if (active && le == k) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sR[c][li] += v[c];
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace msod

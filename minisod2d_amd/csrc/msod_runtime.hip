@@ -3,7 +3,10 @@
 // the reference driver, src/app/sources/main.f90:197-425), the element-major
 // layouts derived once per mesh, table validation and the kernel calls.
 #include "msod2d.h"
+#include "msod_closure.h"
 #include "msod_kernels.h"
+#include "msod_mlp.h"
+#include "msod_plan.h"
 
 #include <algorithm>
 #include <cmath>
@@ -14,6 +17,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <atomic>
 #include <vector>
 
@@ -173,7 +177,7 @@ struct DerivedGeo {
     void *geo, *gvol;
 };
 
-constexpr int kNumTimers = 6;
+constexpr int kNumTimers = 7;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -196,6 +200,9 @@ struct State {
     int64_t launches[kNumTimers] = {0};
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
+    // library-owned device buffers of the closure calls (grow-only)
+    void *scr[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t scr_bytes[4] = {0, 0, 0, 0};
 };
 
 State g;
@@ -340,7 +347,7 @@ int get_tables(int porder, const int32_t *invAtoIJK, const int32_t *AtoI, const 
                                 d + 1, n + 1);
     }
     for (TableSet *ts : g.tables)
-        if (ts->porder == porder && ts->l2g == l2g &&
+        if (ts->porder == porder && ts->l2g == l2g && ts->D.size() == D.size() &&
             std::memcmp(ts->D.data(), D.data(), sizeof(double) * q * q) == 0) {
             *out = ts;
             return MSOD_OK;
@@ -360,24 +367,6 @@ int get_tables(int porder, const int32_t *invAtoIJK, const int32_t *AtoI, const 
 
 constexpr size_t kMaxDerived = 4;
 
-template <class F>
-void parallel_for(int64_t n, F f)
-{
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const unsigned nt = std::min<unsigned>(16, hw);
-    if (n < 4096 || nt == 1) {
-        f(int64_t(0), n);
-        return;
-    }
-    std::vector<std::thread> th;
-    const int64_t chunk = (n + nt - 1) / nt;
-    for (unsigned i = 0; i < nt; ++i) {
-        const int64_t lo = i * chunk, hi = std::min<int64_t>(n, lo + chunk);
-        if (lo < hi) th.emplace_back(f, lo, hi);
-    }
-    for (auto &t : th) t.join();
-}
-
 template <class V>
 int upload(const std::vector<V> &h, V **d, const char *what)
 {
@@ -390,119 +379,30 @@ int upload(const std::vector<V> &h, V **d, const char *what)
     return MSOD_OK;
 }
 
-// Builds the assembly plan of a mesh from its connectivity (device array in
-// the reference layout connec(nelem,nnode), 1-based).  Host work, once per
-// mesh: batches of consecutive elements, their sorted distinct nodes, a greedy
-// colouring of the batches (no two batches of one colour share a node), and
-// the first-writer flag of every (batch, node).
+// Builds the assembly plan of a mesh (msod_plan.h) from its connectivity,
+// a device array in the reference layout, and uploads it.
 int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet *ts, PlanEntry &P)
 {
-    const int nn = (int)ts->l2g.size();
-    const int64_t E = nelem * nn;
-    if (npoin > 0x7fffffff || E > 0xffffffffLL)
-        return fail(MSOD_ERR_MESH, "mesh too large for 32-bit node indices");
-    P.B = batch_size(ts->porder);
-    P.nbatch = (nelem + P.B - 1) / P.B;
+    const int64_t E = nelem * (int64_t)ts->l2g.size();
     std::vector<int32_t> raw(E);
     HIPCHK(hipMemcpy(raw.data(), dconnec, sizeof(int32_t) * E, hipMemcpyDeviceToHost),
            "reading connec");
-    std::vector<uint32_t> conn(E);
-    std::atomic<int64_t> bad{0};
-    const int32_t *l2g = ts->l2g.data();
-    parallel_for(nelem, [&](int64_t lo, int64_t hi) {
-        int64_t nb = 0;
-        for (int64_t e = lo; e < hi; ++e)
-            for (int l = 0; l < nn; ++l) {
-                const int32_t v = raw[e + nelem * l2g[l]];
-                if (v < 1 || v > npoin) {
-                    ++nb;
-                    conn[e * nn + l] = 0;
-                } else {
-                    conn[e * nn + l] = (uint32_t)(v - 1);
-                }
-            }
-        bad += nb;
-    });
-    if (bad.load())
-        return fail(MSOD_ERR_MESH, "connec holds %lld node numbers outside 1..npoin (npoin=%lld)",
-                    (long long)bad.load(), (long long)npoin);
+    HostPlan H;
+    std::string err;
+    if (!build_host_plan(raw.data(), nelem, npoin, ts->l2g, batch_size(ts->porder), H, &err))
+        return fail(MSOD_ERR_MESH, "%s", err.c_str());
     std::vector<int32_t>().swap(raw);
-
-    // distinct nodes per batch
-    const int B = P.B;
-    const int64_t NB = P.nbatch;
-    std::vector<uint32_t> ucount(NB);
-    parallel_for(NB, [&](int64_t lo, int64_t hi) {
-        std::vector<uint32_t> tmp;
-        for (int64_t b = lo; b < hi; ++b) {
-            const int64_t e0 = b * B, e1 = std::min<int64_t>(nelem, e0 + B);
-            tmp.assign(conn.begin() + e0 * nn, conn.begin() + e1 * nn);
-            std::sort(tmp.begin(), tmp.end());
-            ucount[b] = (uint32_t)(std::unique(tmp.begin(), tmp.end()) - tmp.begin());
-        }
-    });
-    std::vector<uint32_t> uoff(NB + 1);
-    uoff[0] = 0;
-    for (int64_t b = 0; b < NB; ++b) uoff[b + 1] = uoff[b] + ucount[b];
-    std::vector<uint32_t>().swap(ucount);
-    std::vector<uint32_t> unode(uoff[NB]);
-    std::vector<uint16_t> lidx(E);
-    parallel_for(NB, [&](int64_t lo, int64_t hi) {
-        for (int64_t b = lo; b < hi; ++b) {
-            const int64_t e0 = b * B, e1 = std::min<int64_t>(nelem, e0 + B);
-            uint32_t *u = unode.data() + uoff[b];
-            std::copy(conn.begin() + e0 * nn, conn.begin() + e1 * nn, u);
-            std::sort(u, u + (e1 - e0) * nn);
-            const uint32_t U = uoff[b + 1] - uoff[b];
-            std::unique(u, u + (e1 - e0) * nn);
-            for (int64_t k = e0 * nn; k < e1 * nn; ++k)
-                lidx[k] = (uint16_t)(std::lower_bound(u, u + U, conn[k]) - u);
-        }
-    });
-
-    // greedy colouring: colour bit c of mask[node] is set once a batch of
-    // colour c holds the node
-    std::vector<uint32_t> mask(npoin, 0u);
-    std::vector<uint8_t> color(NB);
-    int ncolors = 0;
-    for (int64_t b = 0; b < NB; ++b) {
-        uint32_t forb = 0;
-        for (uint32_t k = uoff[b]; k < uoff[b + 1]; ++k) forb |= mask[unode[k]];
-        if (forb == 0xffffffffu) return fail(MSOD_ERR_MESH, "mesh needs more than 32 batch colours");
-        const int c = __builtin_ctz(~forb);
-        color[b] = (uint8_t)c;
-        ncolors = std::max(ncolors, c + 1);
-        for (uint32_t k = uoff[b]; k < uoff[b + 1]; ++k) mask[unode[k]] |= 1u << c;
-    }
-    // first writer of a node: the batch of the lowest colour holding it
-    parallel_for(NB, [&](int64_t lo, int64_t hi) {
-        for (int64_t b = lo; b < hi; ++b) {
-            const uint32_t lower = (1u << color[b]) - 1u;
-            for (uint32_t k = uoff[b]; k < uoff[b + 1]; ++k)
-                if ((mask[unode[k]] & lower) == 0u) unode[k] |= 0x80000000u;
-        }
-    });
-    std::vector<uint32_t> unref;
-    for (int64_t i = 0; i < npoin; ++i)
-        if (mask[i] == 0u) unref.push_back((uint32_t)i);
-    std::vector<uint32_t>().swap(mask);
-    // batches grouped by colour (stable)
-    P.color_off.assign(ncolors + 1, 0);
-    for (int64_t b = 0; b < NB; ++b) P.color_off[color[b] + 1]++;
-    for (int c = 0; c < ncolors; ++c) P.color_off[c + 1] += P.color_off[c];
-    std::vector<uint32_t> order(NB);
-    {
-        std::vector<int64_t> pos(P.color_off.begin(), P.color_off.end() - 1);
-        for (int64_t b = 0; b < NB; ++b) order[pos[color[b]]++] = (uint32_t)b;
-    }
+    P.B = H.B;
+    P.nbatch = H.nbatch;
+    P.color_off = H.color_off;
     int st;
-    if ((st = upload(conn, &P.conn, "plan upload"))) return st;
-    if ((st = upload(lidx, &P.lidx, "plan upload"))) return st;
-    if ((st = upload(uoff, &P.uoff, "plan upload"))) return st;
-    if ((st = upload(unode, &P.unode, "plan upload"))) return st;
-    if ((st = upload(order, &P.order, "plan upload"))) return st;
-    if ((st = upload(unref, &P.unref, "plan upload"))) return st;
-    P.nunref = (int64_t)unref.size();
+    if ((st = upload(H.conn, &P.conn, "plan upload"))) return st;
+    if ((st = upload(H.lidx, &P.lidx, "plan upload"))) return st;
+    if ((st = upload(H.uoff, &P.uoff, "plan upload"))) return st;
+    if ((st = upload(H.unode, &P.unode, "plan upload"))) return st;
+    if ((st = upload(H.order, &P.order, "plan upload"))) return st;
+    if ((st = upload(H.unref, &P.unref, "plan upload"))) return st;
+    P.nunref = (int64_t)H.unref.size();
     return MSOD_OK;
 }
 
@@ -753,7 +653,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, double Cp,
                  double Pr, const T *rho, const T *u, const T *Tem, const T *mu_fluid,
                  const T *mu_e, const T *mu_sgs, double *Rmass, double *Rmom, double *Rener,
-                 int timer)
+                 int timer, bool closure = false, const float *dmlp = nullptr)
 {
     NEED_INIT();
     int st = check_sizes(porder, nelem, npoin);
@@ -762,7 +662,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     const int64_t nn = (int64_t)qn * qn * qn, NE = nelem, NP = npoin;
     TableSet *ts;
     if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts))) return st;
-    const void *dc, *dHe, *dV, *dr, *du, *dT, *dmf, *dme, *dms, *dRm, *dRmo, *dRe;
+    const void *dc, *dHe, *dV, *dr, *du, *dT, *dmf, *dme, *dms = nullptr, *dRm, *dRmo, *dRe;
     uint64_t vc, vHe, vV, vx;
     if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
     if ((st = resolve(He, sizeof(T) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
@@ -772,7 +672,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     if ((st = resolve(Tem, sizeof(T) * NP, "Tem", &dT, &vx))) return st;
     if ((st = resolve(mu_fluid, sizeof(T) * NP, "mu_fluid", &dmf, &vx))) return st;
     if ((st = resolve(mu_e, sizeof(T) * NE * nn, "mu_e", &dme, &vx))) return st;
-    if ((st = resolve(mu_sgs, sizeof(T) * NE * nn, "mu_sgs", &dms, &vx))) return st;
+    if (!closure && (st = resolve(mu_sgs, sizeof(T) * NE * nn, "mu_sgs", &dms, &vx))) return st;
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
@@ -801,10 +701,136 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     a.Rener = (double *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
     if ((st = run_assembled(a, *plan, timer,
-                            [&](const DiffuArgs<T> &x) { return launch_diffu<T>(porder, x, g.stream); },
+                            [&](const DiffuArgs<T> &x) {
+                                if constexpr (std::is_same<T, double>::value)
+                                    if (dmlp) return launch_diffu_closure(porder, x, dmlp, g.stream);
+                                return launch_diffu<T>(porder, x, g.stream);
+                            },
                             "diffusion kernel launch")))
         return st;
     return finish_call("diffusion kernel");
+}
+
+// --- learned closure -----------------------------------------------------
+enum { kScrWeights = 0, kScrPartial = 1, kScrLoss = 2 };
+
+// Library-owned device buffer `slot` of at least `bytes` (grow-only).
+int scratch(int slot, size_t bytes, void **p)
+{
+    if (g.scr_bytes[slot] < bytes) {
+        HIPCHK(hipStreamSynchronize(g.stream), "closure buffer resize");
+        if (g.scr[slot]) (void)hipFree(g.scr[slot]);
+        g.scr[slot] = nullptr;
+        g.scr_bytes[slot] = 0;
+        HIPCHK(hipMalloc(&g.scr[slot], bytes), "hipMalloc closure buffer");
+        g.scr_bytes[slot] = bytes;
+    }
+    *p = g.scr[slot];
+    return MSOD_OK;
+}
+
+// Device address of a weight block (present, device, or plain host memory,
+// which is copied into a library buffer) and whether the model is usable:
+// every value finite and the output scale >= 0.  Reads the values on the host.
+int closure_weights(const float *params, const float **dev, bool *usable)
+{
+    *usable = false;
+    *dev = nullptr;
+    if (!params) return MSOD_OK;
+    const size_t bytes = sizeof(float) * kMlpParams;
+    std::vector<float> h(kMlpParams);
+    uintptr_t base;
+    if (find_region(params, &base) || is_device_pointer(params)) {
+        const void *d;
+        uint64_t ver;
+        int st = resolve(params, bytes, "mlp_params", &d, &ver);
+        if (st) return st;
+        HIPCHK(hipMemcpyAsync(h.data(), d, bytes, hipMemcpyDeviceToHost, g.stream), "reading mlp_params");
+        HIPCHK(hipStreamSynchronize(g.stream), "reading mlp_params");
+        *dev = (const float *)d;
+    } else {
+        std::memcpy(h.data(), params, bytes);
+        void *buf;
+        int st = scratch(kScrWeights, bytes, &buf);
+        if (st) return st;
+        HIPCHK(hipMemcpyAsync(buf, h.data(), bytes, hipMemcpyHostToDevice, g.stream), "uploading mlp_params");
+        HIPCHK(hipStreamSynchronize(g.stream), "uploading mlp_params");
+        *dev = (const float *)buf;
+    }
+    bool ok = h[kOScale] >= 0.f;
+    for (float v : h) ok = ok && std::isfinite(v);
+    *usable = ok;
+    return MSOD_OK;
+}
+
+// Device copy of the l -> Gmsh node map of invAtoIJK (cached with the tables).
+int get_l2g(int porder, const int32_t *invAtoIJK, const int32_t **dl2g)
+{
+    const int q = porder + 1, nn = q * q * q;
+    std::vector<char> tmp;
+    const char *pinv;
+    int st = host_view(invAtoIJK, sizeof(int32_t) * nn, tmp, &pinv, "invAtoIJK");
+    if (st) return st;
+    const int32_t *inv = (const int32_t *)pinv;
+    std::vector<int32_t> l2g(nn);
+    std::vector<char> seen(nn, 0);
+    for (int l = 0; l < nn; ++l) {
+        const int32_t a = inv[l];
+        if (a < 1 || a > nn || seen[a - 1])
+            return fail(MSOD_ERR_TABLE, "invAtoIJK is not a permutation of 1..%d", nn);
+        seen[a - 1] = 1;
+        l2g[l] = a - 1;
+    }
+    for (TableSet *ts : g.tables)
+        if (ts->porder == porder && ts->l2g == l2g) {
+            *dl2g = ts->dl2g;
+            return MSOD_OK;
+        }
+    TableSet *ts = new TableSet;  // map only: D stays empty
+    ts->porder = porder;
+    ts->l2g = l2g;
+    g.tables.push_back(ts);
+    HIPCHK(hipMalloc(&ts->dl2g, sizeof(int32_t) * nn), "hipMalloc tables");
+    HIPCHK(hipMemcpy(ts->dl2g, l2g.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice), "upload l2g");
+    *dl2g = ts->dl2g;
+    return MSOD_OK;
+}
+
+// Shared front half of the closure's element kernels: tables, arrays, plan
+// (for the 0-based connectivity) and element-major geometry.
+int closure_elem_common(int porder, int nelem, int npoin, const int32_t *connec, const double *He,
+                        const double *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                        const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK,
+                        const double *rho, const double *u, ClosureElemArgs &a)
+{
+    NEED_INIT();
+    int st = check_sizes(porder, nelem, npoin);
+    if (st) return st;
+    const int qn = porder + 1;
+    const int64_t nn = (int64_t)qn * qn * qn, NE = nelem, NP = npoin;
+    TableSet *ts;
+    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts))) return st;
+    const void *dc, *dHe, *dV, *dr, *du;
+    uint64_t vc, vHe, vV, vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
+    if ((st = resolve(He, sizeof(double) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
+    if ((st = resolve(gpvol, sizeof(double) * NE * nn, "gpvol", &dV, &vV))) return st;
+    if ((st = resolve(rho, sizeof(double) * NP, "rho", &dr, &vx))) return st;
+    if ((st = resolve(u, sizeof(double) * 3 * NP, "u", &du, &vx))) return st;
+    const PlanEntry *plan;
+    const double *geo, *gvol;
+    if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
+    if ((st = get_geo<double>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+    a = ClosureElemArgs{};
+    a.nelem = NE;
+    a.npoin = NP;
+    a.conn = plan->conn;
+    a.geo = geo;
+    a.gvol = gvol;
+    a.rho = (const double *)dr;
+    a.u = (const double *)du;
+    a.tab = Tables{ts->dD, ts->dl2g};
+    return MSOD_OK;
 }
 
 }  // namespace
@@ -848,6 +874,8 @@ int msod_finalize(void)
     g.tables.clear();
     for (hipEvent_t e : g.free_events) (void)hipEventDestroy(e);
     g.free_events.clear();
+    for (void *p : g.scr)
+        if (p) (void)hipFree(p);
     (void)hipStreamDestroy(g.stream);
     g = State{};
     return MSOD_OK;
@@ -1101,6 +1129,190 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
     return diffu_common<float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                mu_sgs, Rmass, Rmom, Rener, 4);
+}
+
+int msod_full_diffusion_ijk_closure(int porder, int nelem, int npoin, const int32_t *connec,
+                                    const double *Ngp, const double *dNgp, const double *He,
+                                    const double *gpvol, const double *dlxigp_ip,
+                                    const double *xgp, const int32_t *invAtoIJK,
+                                    const int32_t *gmshAtoI, const int32_t *gmshAtoJ,
+                                    const int32_t *gmshAtoK, double Cp, double Pr,
+                                    const double *rho, const double *u, const double *Tem,
+                                    const double *mu_fluid, const double *mu_e,
+                                    const float *mlp_params, int *used_model, double *Rmass,
+                                    double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    (void)Ngp; (void)dNgp; (void)xgp;
+    NEED_INIT();
+    if (used_model) *used_model = 0;
+    const float *dmlp = nullptr;
+    bool usable = false;
+    int st = closure_weights(mlp_params, &dmlp, &usable);
+    if (st) return st;
+    if (used_model) *used_model = usable ? 1 : 0;
+    // unusable model: mu_sgs = 0 (the plain kernel with no mu_sgs array)
+    return diffu_common<double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
+                                nullptr, Rmass, Rmom, Rener, 5, true, usable ? dmlp : nullptr);
+}
+
+int msod_closure_features(int porder, int nelem, int npoin, const int32_t *connec,
+                          const double *He, const double *gpvol, const double *dlxigp_ip,
+                          const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                          const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const double *rho,
+                          const double *u, float *features)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ClosureElemArgs a;
+    int st = closure_elem_common(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, rho, u, a);
+    if (st) return st;
+    const int64_t nn = (int64_t)(porder + 1) * (porder + 1) * (porder + 1);
+    const void *df;
+    uint64_t vx;
+    if ((st = resolve(features, sizeof(float) * kMlpIn * nn * nelem, "features", &df, &vx))) return st;
+    a.feat = (float *)df;
+    TimedLaunch tk(6);
+    HIPCHK(launch_closure_features(porder, a, g.stream), "closure features launch");
+    tk.done();
+    return finish_call("closure features");
+}
+
+int msod_closure_predict(int porder, int nelem, const int32_t *invAtoIJK, const float *mlp_params,
+                         const float *features, double *mu_sgs)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    int st = check_sizes(porder, nelem, 1);
+    if (st) return st;
+    const int nn = (porder + 1) * (porder + 1) * (porder + 1);
+    const int32_t *dl2g;
+    if ((st = get_l2g(porder, invAtoIJK, &dl2g))) return st;
+    const float *dmlp;
+    bool usable;
+    if ((st = closure_weights(mlp_params, &dmlp, &usable))) return st;
+    if (!usable) return fail(MSOD_ERR_MODEL, "mlp_params missing, not finite or with a negative scale");
+    const void *df, *dm;
+    uint64_t vx;
+    if ((st = resolve(features, sizeof(float) * kMlpIn * nn * (int64_t)nelem, "features", &df, &vx))) return st;
+    if ((st = resolve(mu_sgs, sizeof(double) * nn * (int64_t)nelem, "mu_sgs", &dm, &vx))) return st;
+    TimedLaunch tk(6);
+    HIPCHK(launch_mlp_predict(dmlp, (const float *)df, nelem, nn, dl2g, (double *)dm, g.stream),
+           "closure predict launch");
+    tk.done();
+    return finish_call("closure predict");
+}
+
+int msod_closure_sensitivity(int porder, int nelem, int npoin, const int32_t *connec,
+                             const double *He, const double *gpvol, const double *dlxigp_ip,
+                             const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                             const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                             const double *rho, const double *u, const double *Tem,
+                             const double *rerr, float *gnu)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    ClosureElemArgs a;
+    int st = closure_elem_common(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, rho, u, a);
+    if (st) return st;
+    const int64_t nn = (int64_t)(porder + 1) * (porder + 1) * (porder + 1);
+    const void *dT, *de, *dg;
+    uint64_t vx;
+    if ((st = resolve(Tem, sizeof(double) * npoin, "Tem", &dT, &vx))) return st;
+    if ((st = resolve(rerr, sizeof(double) * 4 * (int64_t)npoin, "rerr", &de, &vx))) return st;
+    if ((st = resolve(gnu, sizeof(float) * nn * nelem, "gnu", &dg, &vx))) return st;
+    a.Tem = (const double *)dT;
+    a.Cp = Cp;
+    a.rerr = (const double *)de;
+    a.gnu = (float *)dg;
+    TimedLaunch tk(6);
+    HIPCHK(launch_closure_adjoint(porder, a, g.stream), "closure sensitivity launch");
+    tk.done();
+    return finish_call("closure sensitivity");
+}
+
+int msod_closure_gradient(int64_t nsamples, const float *mlp_params, const float *features,
+                          const float *gnu, float *grad)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (nsamples <= 0) return fail(MSOD_ERR_ARG, "nsamples=%lld", (long long)nsamples);
+    const float *dmlp;
+    bool usable;
+    int st = closure_weights(mlp_params, &dmlp, &usable);
+    if (st) return st;
+    if (!usable) return fail(MSOD_ERR_MODEL, "mlp_params missing, not finite or with a negative scale");
+    const void *df, *dg, *dgr;
+    uint64_t vx;
+    if ((st = resolve(features, sizeof(float) * kMlpIn * nsamples, "features", &df, &vx))) return st;
+    if ((st = resolve(gnu, sizeof(float) * nsamples, "gnu", &dg, &vx))) return st;
+    if ((st = resolve(grad, sizeof(float) * kMlpTrain, "grad", &dgr, &vx))) return st;
+    const int nblocks = (int)std::min<int64_t>((nsamples + 63) / 64, 1024);
+    void *part;
+    if ((st = scratch(kScrPartial, sizeof(float) * kMlpTrain * nblocks, &part))) return st;
+    TimedLaunch tk(6);
+    HIPCHK(launch_mlp_grad(dmlp, (const float *)df, (const float *)dg, nsamples, (float *)part, nblocks,
+                           (float *)dgr, g.stream),
+           "closure gradient launch");
+    tk.done();
+    return finish_call("closure gradient");
+}
+
+int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, int step,
+                      double lr, double beta1, double beta2, double eps)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (step < 1 || !(lr > 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0))
+        return fail(MSOD_ERR_ARG, "adam: step=%d lr=%g beta1=%g beta2=%g eps=%g", step, lr, beta1, beta2, eps);
+    const void *dp, *dg, *dm, *dv;
+    uint64_t vx;
+    int st;
+    if ((st = resolve(mlp_params, sizeof(float) * kMlpParams, "mlp_params", &dp, &vx))) return st;
+    if ((st = resolve(grad, sizeof(float) * kMlpTrain, "grad", &dg, &vx))) return st;
+    if ((st = resolve(m, sizeof(float) * kMlpTrain, "m", &dm, &vx))) return st;
+    if ((st = resolve(v, sizeof(float) * kMlpTrain, "v", &dv, &vx))) return st;
+    TimedLaunch tk(6);
+    HIPCHK(launch_adam((float *)dp, (const float *)dg, (float *)dm, (float *)dv, step, (float)lr,
+                       (float)beta1, (float)beta2, (float)eps, g.stream),
+           "adam launch");
+    tk.done();
+    return finish_call("adam");
+}
+
+int msod_closure_loss(int npoin, const double *Rmom, const double *Rener, const double *target,
+                      const double *weights, double *rerr, double *loss)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (npoin <= 0 || !weights || !loss) return fail(MSOD_ERR_ARG, "npoin=%d, weights and loss needed", npoin);
+    const int64_t NP = npoin;
+    const void *dmo, *den, *dt, *de;
+    uint64_t vx;
+    int st;
+    if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &den, &vx))) return st;
+    if ((st = resolve(target, sizeof(double) * 4 * NP, "target", &dt, &vx))) return st;
+    if ((st = resolve(rerr, sizeof(double) * 4 * NP, "rerr", &de, &vx))) return st;
+    const int nblocks = (int)std::min<int64_t>((NP + 255) / 256, 1024);
+    void *buf;
+    if ((st = scratch(kScrLoss, sizeof(double) * (4 + nblocks), &buf))) return st;
+    double *dw = (double *)buf, *lpart = dw + 4;
+    std::vector<double> part(nblocks);
+    HIPCHK(hipMemcpyAsync(dw, weights, sizeof(double) * 4, hipMemcpyHostToDevice, g.stream), "loss weights");
+    TimedLaunch tk(6);
+    HIPCHK(launch_residual_error((const double *)dmo, (const double *)den, (const double *)dt, dw, NP,
+                                 (double *)de, lpart, nblocks, g.stream),
+           "closure loss launch");
+    tk.done();
+    HIPCHK(hipMemcpyAsync(part.data(), lpart, sizeof(double) * nblocks, hipMemcpyDeviceToHost, g.stream),
+           "loss partial sums");
+    HIPCHK(hipStreamSynchronize(g.stream), "closure loss");
+    double sum = 0.0;
+    for (double x : part) sum += x;
+    *loss = sum;
+    return finish_call("closure loss");
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@ module msod_capi
    use msod_mesh, only: msod_box_mesh, msod_disjoint_mesh, msod_box_sizes
    use msod_geom, only: msod_elem_jacobian
    use msod_fields, only: msod_flow_fields, msod_viscosity_fields
+   use msod_interp, only: msod_interp_1d, msod_prolong_fields, msod_restrict_residual
    implicit none
 
 contains
@@ -103,5 +104,31 @@ contains
       real(c_double), intent(out) :: mu_e(nelem,ngaus), mu_sgs(nelem,ngaus)
       call msod_viscosity_fields(nelem, ngaus, seed_mue, seed_musgs, mue_max, musgs_max, mu_e, mu_sgs)
    end subroutine msodh_viscosity_fields
+
+   subroutine msodh_interp_1d(pf, pt, M) bind(C, name='msodh_interp_1d')
+      integer(c_int), value :: pf, pt
+      real(c_double), intent(out) :: M(pt+1, pf+1)
+      real(8) :: Mt(pt+1, pf+1)
+      call msod_interp_1d(pf, pt, Mt)
+      M = Mt
+   end subroutine msodh_interp_1d
+
+   subroutine msodh_prolong_fields(pf, pt, nelem, npf, npt, ncomp, connec_f, connec_t, ff, ft) &
+         bind(C, name='msodh_prolong_fields')
+      integer(c_int), value :: pf, pt, nelem, npf, npt, ncomp
+      integer(c_int), intent(in) :: connec_f(nelem,(pf+1)**3), connec_t(nelem,(pt+1)**3)
+      real(c_double), intent(in) :: ff(npf, ncomp)
+      real(c_double), intent(out) :: ft(npt, ncomp)
+      call msod_prolong_fields(pf, pt, nelem, npf, npt, ncomp, connec_f, connec_t, ff, ft)
+   end subroutine msodh_prolong_fields
+
+   subroutine msodh_restrict_residual(pf, pt, nelem, npf, npt, ncomp, connec_f, connec_t, rf, rt) &
+         bind(C, name='msodh_restrict_residual')
+      integer(c_int), value :: pf, pt, nelem, npf, npt, ncomp
+      integer(c_int), intent(in) :: connec_f(nelem,(pf+1)**3), connec_t(nelem,(pt+1)**3)
+      real(c_double), intent(in) :: rf(npf, ncomp)
+      real(c_double), intent(out) :: rt(npt, ncomp)
+      call msod_restrict_residual(pf, pt, nelem, npf, npt, ncomp, connec_f, connec_t, rf, rt)
+   end subroutine msodh_restrict_residual
 
 end module msod_capi

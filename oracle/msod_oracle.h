@@ -61,6 +61,18 @@ void msod_oracle_diffu(int p, int nelem, int npoin, const int32_t *connec, const
                        const double *mu_sgs, double *Rmass, double *Rmom, double *Rener,
                        int nthreads);
 
+/* Learned closure (msod_oracle_mlp.c): the MLP mu_sgs = scale*softplus(z3)
+ * of a 1530-float parameter block, per sample of 12 features; quant = 1
+ * rounds to bfloat16 where the GPU feeds its matrix cores. */
+void msod_oracle_mlp_forward(const float *P, const float *features, int64_t n, int quant,
+                             double *z3, double *mu_sgs);
+/* grad[0..1504] = sum_i gnu[i] d mu_sgs_i / d P[j] */
+void msod_oracle_mlp_gradient(const float *P, const float *features, const double *gnu, int64_t n,
+                              int quant, double *grad);
+/* one Adam step on P[0..1504] */
+void msod_oracle_adam(float *P, const float *grad, float *m, float *v, int step, double lr,
+                      double beta1, double beta2, double eps);
+
 #ifdef __cplusplus
 }
 #endif

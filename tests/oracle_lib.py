@@ -33,6 +33,9 @@ class Oracle:
         L = ctypes.CDLL(ORACLE_SO, mode=ctypes.RTLD_LOCAL)
         L.msod_oracle_convec.argtypes = [ctypes.c_int] * 3 + [_vp] * 16 + [ctypes.c_int]
         L.msod_oracle_diffu.argtypes = [ctypes.c_int] * 3 + [_vp] * 8 + [_d] * 4 + [_vp] * 9 + [ctypes.c_int]
+        L.msod_oracle_mlp_forward.argtypes = [_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp, _vp]
+        L.msod_oracle_mlp_gradient.argtypes = [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, _vp]
+        L.msod_oracle_adam.argtypes = [_vp] * 4 + [ctypes.c_int, _d, _d, _d, _d]
         self.L = L
 
     def gll(self, p, exact=False):
@@ -73,6 +76,33 @@ class Oracle:
                 prob.rho, prob.u, prob.Tem, prob.mu_fluid, prob.mu_e,
                 prob.mu_sgs if mu_sgs is None else mu_sgs)), *map(_p, R), nthreads)
         return R
+
+
+    # ---- learned closure (oracle/msod_oracle_mlp.c) ---------------------------
+    def mlp_forward(self, P, feat, quant=1):
+        """(z3, mu_sgs) per sample."""
+        P = np.ascontiguousarray(P, np.float32)
+        feat = np.ascontiguousarray(feat, np.float32)
+        n = feat.shape[0]
+        z3, mu = np.zeros(n), np.zeros(n)
+        self.L.msod_oracle_mlp_forward(P.ctypes.data, feat.ctypes.data, n, quant, z3.ctypes.data,
+                                       mu.ctypes.data)
+        return z3, mu
+
+    def mlp_gradient(self, P, feat, gnu, quant=1):
+        P = np.ascontiguousarray(P, np.float32)
+        feat = np.ascontiguousarray(feat, np.float32)
+        gnu = np.ascontiguousarray(gnu, np.float64)
+        g = np.zeros(1505)
+        self.L.msod_oracle_mlp_gradient(P.ctypes.data, feat.ctypes.data, gnu.ctypes.data,
+                                        feat.shape[0], quant, g.ctypes.data)
+        return g
+
+    def adam(self, P, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+        """In place on float32 arrays P, m, v."""
+        grad = np.ascontiguousarray(grad, np.float32)
+        self.L.msod_oracle_adam(P.ctypes.data, grad.ctypes.data, m.ctypes.data, v.ctypes.data,
+                                step, lr, b1, b2, eps)
 
 
 def ref_available(p):
@@ -140,3 +170,4 @@ class RefLib:
 def rel_l2(a, b):
     nb = np.linalg.norm(np.ravel(b))
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / (nb if nb > 0 else 1.0))
+
