@@ -9,7 +9,7 @@ already resident in HBM.
 
 Prints one JSON line: GLL-node updates/s = npoin / (t_convec + t_diffu), the
 roofline of the dominant kernel from HIP-event durations and the canonical
-algorithmic bytes, the mixed-precision variant beside it, a sampled-node
+algorithmic bytes, the FP32-storage variants beside it, a sampled-node
 parity check against the CPU oracle, and the reference's own CPU kernels timed
 on a bounded sample of the same workload.
 """
@@ -35,18 +35,24 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def canonical_bytes(E, N, wg, wf):
+def canonical_bytes(E, N, wg, wf, we=None):
     """Algorithmic bytes of one convective and one diffusive call: each input
-    element read once, each output written once (FP64).  wg/wf: bytes per
-    geometry/field word."""
+    element read once, each output written once (FP64).  wg/wf/we: bytes per
+    geometry word, nodal-field word and element-field word (mu_e, mu_sgs)."""
+    we = wf if we is None else we
     bc = E * (4 + 10 * wg) + N * (9 * wf + 40)
-    bd = E * (4 + 10 * wg + 2 * wf) + N * (6 * wf + 40)
+    bd = E * (4 + 10 * wg + 2 * we) + N * (6 * wf + 40)
     return bc, bd
 
 
-def run_pair(prob, R, D, mixed, f32=None):
+TIMERS = {"fp64": (m.TIMER_CONVEC, m.TIMER_DIFFU),
+          "mixed": (m.TIMER_CONVEC_MIXED, m.TIMER_DIFFU_MIXED),
+          "geo32": (m.TIMER_CONVEC_GEO32, m.TIMER_DIFFU_GEO32)}
+
+
+def run_pair(prob, R, D, variant, f32=None):
     t = prob.tables
-    if mixed:
+    if variant == "mixed":
         m.full_convec_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
                                 t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, f32["u"],
                                 f32["q"], f32["rho"], f32["E"], prob.gamma, *R)
@@ -54,32 +60,40 @@ def run_pair(prob, R, D, mixed, f32=None):
                                    t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp,
                                    prob.Pr, f32["rho"], f32["u"], f32["Tem"], f32["mu_fluid"],
                                    f32["mu_e"], f32["mu_sgs"], *D)
+    elif variant == "geo32":
+        m.full_convec_ijk_geo32(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
+                                t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.u, prob.q,
+                                prob.rho, prob.pr, prob.E, *R)
+        m.full_diffusion_ijk_geo32(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
+                                   t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp,
+                                   prob.Pr, prob.rho, prob.u, prob.Tem, prob.mu_fluid,
+                                   f32["mu_e"], f32["mu_sgs"], *D)
     else:
         m.full_convec_ijk(*prob.convec_args(*R))
         m.full_diffusion_ijk(*prob.diffu_args(*D))
 
 
-def timed(prob, R, D, steps, warmup, mixed, f32=None):
+def timed(prob, R, D, steps, warmup, variant, f32=None):
     m.set_async(True)
     for _ in range(warmup):
-        run_pair(prob, R, D, mixed, f32)
+        run_pair(prob, R, D, variant, f32)
     m.synchronize()
     m.timing_reset()
     m.timing_enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
-        run_pair(prob, R, D, mixed, f32)
+        run_pair(prob, R, D, variant, f32)
     m.synchronize()
     t1 = time.perf_counter()
     m.timing_enable(False)
     m.set_async(False)
-    kc, kd = (m.TIMER_CONVEC_MIXED, m.TIMER_DIFFU_MIXED) if mixed else (m.TIMER_CONVEC, m.TIMER_DIFFU)
+    kc, kd = TIMERS[variant]
     zc = m.timing_get(m.TIMER_ZERO)
     c = m.timing_get(kc)
     d = m.timing_get(kd)
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps,
                 convec_ms=c[0] / max(c[1], 1), diffu_ms=d[0] / max(d[1], 1),
-                zero_ms=zc[0] / max(zc[1], 1))
+                zero_ms=zc[0] / max(zc[1], 1), launches_per_call=c[1] / max(steps, 1))
 
 
 def sampled_parity(prob, out, nsample=3000, seed=7):
@@ -156,7 +170,7 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads", type=int, default=16)
-    ap.add_argument("--assembly", choices=("colored", "atomic"), default="colored",
+    ap.add_argument("--assembly", choices=("colored", "atomic"), default="atomic",
                     help="node assembly of the reported run")
     ap.add_argument("--compare-assembly", action="store_true",
                     help="also time the other assembly mode (FP64)")
@@ -184,10 +198,10 @@ def main():
     other = None
     if args.compare_assembly:
         m.set_assembly(m.ASSEMBLY_COLORED if mode == m.ASSEMBLY_ATOMIC else m.ASSEMBLY_ATOMIC)
-        other = timed(prob, R, D, args.steps, args.warmup, mixed=False)
+        other = timed(prob, R, D, args.steps, args.warmup, "fp64")
         log(f"FP64 other assembly {other}")
     m.set_assembly(mode)
-    r64 = timed(prob, R, D, args.steps, args.warmup, mixed=False)
+    r64 = timed(prob, R, D, args.steps, args.warmup, "fp64")
     log(f"FP64 {r64}")
     parity = None
     if not args.no_parity:
@@ -195,16 +209,18 @@ def main():
         parity = sampled_parity(prob, R + D)
         log(f"sampled parity {parity}")
 
-    rmx = None
+    split = {}
     if not args.no_mixed:
         f32 = prob.fp32()
         m.copyin(*[f32[k] for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")])
-        log("timing mixed precision")
-        rmx = timed(prob, R, D, args.steps, args.warmup, mixed=True, f32=f32)
-        log(f"mixed {rmx}")
-        if parity is not None:
-            m.update_host(*R, *D)
-            rmx["sampled_rel_l2"] = sampled_parity(prob, R + D)
+        for variant in ("mixed", "geo32"):
+            log(f"timing {variant}")
+            r = timed(prob, R, D, args.steps, args.warmup, variant, f32=f32)
+            log(f"{variant} {r}")
+            if parity is not None:
+                m.update_host(*R, *D)
+                r["sampled_rel_l2"] = sampled_parity(prob, R + D)
+            split[variant] = r
 
     bc, bd = canonical_bytes(E, N, 8, 8)
     dom, dom_ms, dom_b = ("full_convec_ijk", r64["convec_ms"], bc)
@@ -237,14 +253,20 @@ def main():
                                  "ms_per_step": other["ms_per_step"],
                                  "kernels_ms": {"convec": other["convec_ms"], "diffu": other["diffu_ms"],
                                                 "zero_fill": other["zero_ms"]}}
-    if rmx is not None:
-        bcm, bdm = canonical_bytes(E, N, 4, 4)
-        rec["mixed"] = {"value": N / (rmx["ms_per_step"] * 1e-3), "ms_per_step": rmx["ms_per_step"],
-                        "speedup_vs_fp64": r64["ms_per_step"] / rmx["ms_per_step"],
-                        "kernels_ms": {"convec": rmx["convec_ms"], "diffu": rmx["diffu_ms"]},
-                        "hbm_frac_convec": bcm / (rmx["convec_ms"] * 1e-3) / HBM_PEAK,
-                        "hbm_frac_diffu": bdm / (rmx["diffu_ms"] * 1e-3) / HBM_PEAK,
-                        "rel_l2_vs_fp64_oracle": rmx.get("sampled_rel_l2")}
+    notes = {"mixed": "every input stored in FP32, FP64 arithmetic and assembly; error = FP32 "
+                      "rounding of the inputs, amplified by the derivative sweeps",
+             "geo32": "He, gpvol, mu_e, mu_sgs stored in FP32, nodal fields FP64, FP64 "
+                      "arithmetic and assembly"}
+    for variant, r in split.items():
+        wb = (4, 4, 4) if variant == "mixed" else (4, 8, 4)
+        bcm, bdm = canonical_bytes(E, N, *wb)
+        rec[variant] = {"value": N / (r["ms_per_step"] * 1e-3), "ms_per_step": r["ms_per_step"],
+                        "speedup_vs_fp64": r64["ms_per_step"] / r["ms_per_step"],
+                        "kernels_ms": {"convec": r["convec_ms"], "diffu": r["diffu_ms"]},
+                        "hbm_frac_convec": bcm / (r["convec_ms"] * 1e-3) / HBM_PEAK,
+                        "hbm_frac_diffu": bdm / (r["diffu_ms"] * 1e-3) / HBM_PEAK,
+                        "rel_l2_vs_fp64_oracle": r.get("sampled_rel_l2"),
+                        "storage": notes[variant]}
     m.finalize()
     del prob, R, D, ins
     if not args.no_cpu_baseline:

@@ -72,12 +72,15 @@ int msod_data_invalidate(const void *ptr);
 
 /* ---- execution control -------------------------------------------------- */
 /* How the element contributions meet at shared nodes (no counterpart in the
- * reference, which uses `!$acc atomic update`, elem_convec.f90:414-425):
- *   MSOD_ASSEMBLY_COLORED (default): per-batch node sums in LDS, batches
- *     coloured so that one launch per colour writes each node without atomics;
- *     the summation order is fixed, results are bitwise reproducible.
- *   MSOD_ASSEMBLY_ATOMIC: the same per-batch sums, FP64 atomics into zeroed
- *     outputs (one launch). */
+ * reference, which uses `!$acc atomic update`, elem_convec.f90:414-425).
+ * Both modes first sum the contributions of a batch of elements per node in
+ * LDS, in a fixed order.
+ *   MSOD_ASSEMBLY_ATOMIC (default): one launch over all batches in mesh
+ *     order, FP64 atomics of the batch sums into zeroed outputs; the fastest
+ *     (neighbouring batches meet in the same L2).
+ *   MSOD_ASSEMBLY_COLORED: batches coloured so that one launch per colour
+ *     writes each node without atomics; the summation order is fixed and the
+ *     results are bitwise reproducible from run to run. */
 enum { MSOD_ASSEMBLY_COLORED = 0, MSOD_ASSEMBLY_ATOMIC = 1 };
 int msod_set_assembly(int mode);
 int msod_set_async(int async);
@@ -86,8 +89,8 @@ int msod_synchronize(void);
 int msod_timing_enable(int enable);
 /* kernel: 0 zero-fill, 1 convec, 2 diffu, 3 convec mixed, 4 diffu mixed,
  *         5 diffu learned closure, 6 closure training kernels (features,
- *         predict, sensitivity, gradient, adam, loss).  Returns accumulated ms
- *         and launch count. */
+ *         predict, sensitivity, gradient, adam, loss), 7 convec geo32,
+ *         8 diffu geo32.  Returns accumulated ms and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
 
@@ -118,11 +121,19 @@ int msod_full_diffusion_ijk(int porder, int nelem, int npoin, const int32_t *con
                             double *Rmass, double *Rmom, double *Rener);
 int msod_set_diffusion_coefficients(double c_rho, double c_ener);
 
-/* Mixed-precision forms (no counterpart in the reference, whose single real
- * kind rp is global, mod_constants.f90:5): geometry, fields, fluxes and
- * derivative sweeps in FP32, nodal assembly in FP64 atomics.  The convective
- * form takes no pressure array: it recovers p = (gamma-1)(E - rho|u|^2/2) in
- * FP64 from the FP32 conservative state. */
+/* Precision splits (no counterpart in the reference, whose single real kind
+ * rp is global, mod_constants.f90:5).  Inputs stored in FP32 halve their HBM
+ * bytes; the kernels compute in FP64 (on gfx950 an FP64 FMA issues at the
+ * rate of an unpacked FP32 one) and assemble in FP64, so the only error is
+ * the FP32 rounding of the inputs.
+ *   _mixed: every input in FP32.  The convective form takes no pressure
+ *     array: it recovers p = (gamma-1)(E - rho|u|^2/2) in FP64 from the FP32
+ *     conservative state.
+ *   _geo32: the element-sized arrays (He, gpvol; mu_e, mu_sgs) in FP32, the
+ *     nodal fields in FP64.  The element arrays enter the residuals as factors
+ *     and are never differenced, so their rounding is not amplified by the
+ *     derivative sweeps (relative error about 3e-7 where the FP32 nodal fields
+ *     of _mixed give about 1e-5 at 128^3, growing like 1/h). */
 int msod_full_convec_ijk_mixed(int porder, int nelem, int npoin, const int32_t *connec,
                                const float *He, const float *gpvol, const double *dlxigp_ip,
                                const int32_t *invAtoIJK, const int32_t *gmshAtoI,
@@ -135,6 +146,20 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
                                   const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
                                   double Pr, const float *rho, const float *u, const float *Tem,
                                   const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                  double *Rmass, double *Rmom, double *Rener);
+
+int msod_full_convec_ijk_geo32(int porder, int nelem, int npoin, const int32_t *connec,
+                               const float *He, const float *gpvol, const double *dlxigp_ip,
+                               const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                               const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const double *u,
+                               const double *q, const double *rho, const double *pr,
+                               const double *E, double *Rmass, double *Rmom, double *Rener);
+int msod_full_diffusion_ijk_geo32(int porder, int nelem, int npoin, const int32_t *connec,
+                                  const float *He, const float *gpvol, const double *dlxigp_ip,
+                                  const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                                  const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                                  double Pr, const double *rho, const double *u, const double *Tem,
+                                  const double *mu_fluid, const float *mu_e, const float *mu_sgs,
                                   double *Rmass, double *Rmom, double *Rener);
 
 /* ---- learned subgrid-viscosity closure ----------------------------------

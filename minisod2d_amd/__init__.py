@@ -38,7 +38,8 @@ MSOD_ERR_MESH = 7
 MSOD_ERR_MODEL = 8
 
 # timer ids of msod_timing_get
-TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP = range(6)
+(TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP,
+ TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32) = range(9)
 
 # the C-ABI entry points declared in include/msod2d.h (checked by the tests)
 ABI_SYMBOLS = (
@@ -49,6 +50,7 @@ ABI_SYMBOLS = (
     "msod_synchronize", "msod_timing_enable", "msod_timing_get", "msod_timing_reset",
     "msod_full_convec_ijk", "msod_full_diffusion_ijk", "msod_set_diffusion_coefficients",
     "msod_full_convec_ijk_mixed", "msod_full_diffusion_ijk_mixed",
+    "msod_full_convec_ijk_geo32", "msod_full_diffusion_ijk_geo32",
     "msod_full_diffusion_ijk_closure", "msod_closure_features", "msod_closure_predict",
     "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
     "msod_closure_loss",
@@ -98,6 +100,8 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_full_diffusion_ijk.argtypes = [_i, _i, _i] + [_vp] * 11 + [_d, _d] + [_vp] * 9
         L.msod_full_convec_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 12 + [_d] + [_vp] * 3
         L.msod_full_diffusion_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
+        L.msod_full_convec_ijk_geo32.argtypes = [_i, _i, _i] + [_vp] * 16
+        L.msod_full_diffusion_ijk_geo32.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
         L.msod_full_diffusion_ijk_closure.argtypes = ([_i, _i, _i] + [_vp] * 11 + [_d, _d] +
                                                       [_vp] * 10)
         L.msod_closure_features.argtypes = [_i, _i, _i] + [_vp] * 11
@@ -187,11 +191,13 @@ def present(a: np.ndarray) -> bool:
 
 ASSEMBLY_COLORED = 0
 ASSEMBLY_ATOMIC = 1
+ASSEMBLY_DEFAULT = ASSEMBLY_ATOMIC
 
 
 def set_assembly(mode: int) -> None:
-    """ASSEMBLY_COLORED (default, atomic-free, bitwise reproducible) or
-    ASSEMBLY_ATOMIC (FP64 atomics into zeroed outputs)."""
+    """ASSEMBLY_ATOMIC (default: one launch, FP64 atomics into zeroed outputs,
+    the fastest) or ASSEMBLY_COLORED (one launch per batch colour, no atomics,
+    bitwise reproducible)."""
     _check(hip_lib().msod_set_assembly(mode))
 
 
@@ -251,7 +257,7 @@ def full_diffusion_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xg
 
 def full_convec_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
                           gmshAtoJ, gmshAtoK, u32, q32, rho32, E32, gamma_gas, Rmass, Rmom, Rener):
-    """Mixed-precision convective residual: FP32 state/geometry, FP64 assembly."""
+    """Convective residual from FP32 state and geometry (FP64 arithmetic and assembly)."""
     _check(hip_lib().msod_full_convec_ijk_mixed(
         _porder(invAtoIJK), nelem, npoin,
         *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
@@ -261,12 +267,32 @@ def full_convec_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAto
 def full_diffusion_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
                              gmshAtoJ, gmshAtoK, Cp, Pr, rho32, u32, Tem32, mu_fluid32, mu_e32,
                              mu_sgs32, Rmass, Rmom, Rener):
-    """Mixed-precision diffusive residual: FP32 state/geometry, FP64 assembly."""
+    """Diffusive residual from FP32 state and geometry (FP64 arithmetic and assembly)."""
     _check(hip_lib().msod_full_diffusion_ijk_mixed(
         _porder(invAtoIJK), nelem, npoin,
         *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
         float(Cp), float(Pr),
         *map(_ptr, (rho32, u32, Tem32, mu_fluid32, mu_e32, mu_sgs32, Rmass, Rmom, Rener))))
+
+
+def full_convec_ijk_geo32(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                          gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener):
+    """Convective residual with FP32 geometry and FP64 fields (FP64 arithmetic)."""
+    _check(hip_lib().msod_full_convec_ijk_geo32(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
+                    u, q, rho, pr, E, Rmass, Rmom, Rener))))
+
+
+def full_diffusion_ijk_geo32(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                             gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e32, mu_sgs32,
+                             Rmass, Rmom, Rener):
+    """Diffusive residual with FP32 geometry, mu_e, mu_sgs and FP64 fields."""
+    _check(hip_lib().msod_full_diffusion_ijk_geo32(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
+        float(Cp), float(Pr),
+        *map(_ptr, (rho, u, Tem, mu_fluid, mu_e32, mu_sgs32, Rmass, Rmom, Rener))))
 
 
 # ---------------------------------------------------------------------------

@@ -330,11 +330,18 @@ class _Device:
 
 
 def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: int = 0,
-          init: dict | None = None, log=None) -> dict:
+          init: dict | None = None, log=None, val_pairs: list[ClosurePair] | None = None,
+          eval_every: int = 25) -> dict:
     """Adam on the sum of the pairs' losses, full batch, all on the GPU.
-    Returns params, Adam state and the loss history."""
+
+    With val_pairs, the parameters of the lowest validation loss are kept
+    (early stopping); the mu_sgs = 0 model scores 4 per validation pair.
+    Returns params, Adam state, the training loss history and the
+    validation history [(step, loss)]."""
     devs = [_Device(p) for p in pairs]
+    vdevs = []
     try:
+        vdevs = [_Device(p) for p in (val_pairs or [])]
         if init is not None:
             params = init["params"].copy()
             m, v, step0 = init["adam_m"].copy(), init["adam_v"].copy(), init["step"]
@@ -353,9 +360,22 @@ def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: in
         grad = np.zeros(CLOSURE_NTRAIN, np.float32)
         gtmp = np.zeros(CLOSURE_NTRAIN, np.float32)
         copyin(params, m, v, grad, gtmp)
-        hist = []
+        hist, vhist = [], []
+        best = None
         t0 = time.time()
+
+        def validate(step):
+            nonlocal best
+            Lv = float(sum(d.loss(params, False) for d in vdevs))
+            vhist.append((step, Lv))
+            if best is None or Lv < best[0]:
+                update_host(params)
+                best = (Lv, params.copy(), step)
+            return Lv
+
         try:
+            if vdevs:
+                validate(step0)
             for it in range(1, steps + 1):
                 L = 0.0
                 if len(devs) == 1:
@@ -370,14 +390,22 @@ def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: in
                     update_device(grad)
                 closure_adam(params, grad, m, v, step0 + it, lr)
                 hist.append(L)
+                Lv = validate(step0 + it) if vdevs and (it % eval_every == 0 or it == steps) else None
                 if log and (it == 1 or it % 50 == 0 or it == steps):
-                    log(f"step {step0 + it}: loss {L:.6f} ({time.time() - t0:.1f} s)")
+                    extra = f", validation {Lv:.6f}" if Lv is not None else ""
+                    log(f"step {step0 + it}: loss {L:.6f}{extra} ({time.time() - t0:.1f} s)")
             update_host(params, m, v)
         finally:
             delete(params, m, v, grad, gtmp)
-        return dict(params=params, adam_m=m, adam_v=v, step=step0 + steps, history=hist)
+        out = dict(params=params, adam_m=m, adam_v=v, step=step0 + steps, history=hist,
+                   val_history=vhist, last_params=params.copy())
+        if best is not None:
+            out["params"] = best[1]
+            out["best_step"] = best[2]
+            out["best_val"] = best[0]
+        return out
     finally:
-        for d in devs:
+        for d in devs + vdevs:
             d.close()
 
 

@@ -26,14 +26,14 @@ hipError_t launch_mlp_predict(const float *P, const float *feat, int64_t nelem, 
                               const int32_t *l2g, double *mu_sgs, hipStream_t s);
 hipError_t launch_mlp_grad(const float *P, const float *feat, const float *gnu, int64_t nsamp,
                            float *partial, int nblocks, float *grad, hipStream_t s);
-hipError_t launch_adam(float *P, const float *grad, float *m, float *v, int step, float lr, float b1,
-                       float b2, float eps, hipStream_t s);
+hipError_t launch_adam(float *P, const float *grad, float *m, float *v, int step, double lr, double b1,
+                       double b2, double eps, hipStream_t s);
 hipError_t launch_residual_error(const double *Rmom, const double *Rener, const double *target,
                                  const double *w, int64_t npoin, double *rerr, double *lpart,
                                  int nblocks, hipStream_t s);
 
 // Diffusion with mu_sgs from the MLP, evaluated inside the kernel (FP64).
-hipError_t launch_diffu_closure(int porder, const DiffuArgs<double> &a, const float *mlp,
+hipError_t launch_diffu_closure(int porder, const DiffuArgs<double, double> &a, const float *mlp,
                                 hipStream_t s);
 
 }  // namespace msod

@@ -13,6 +13,8 @@
 // sweep of the heat flux (the same quantities the forward kernel forms).
 // k_adjoint computes g for every element node; k_mlp_grad back-propagates it
 // through the MLP with bf16 MFMA products and FP32 accumulation.
+#include <cmath>
+
 #include "msod_kernels.h"
 #include "msod_mlp.h"
 #include "msod_closure.h"
@@ -349,14 +351,16 @@ __global__ void k_grad_reduce(const float *partial, int nblocks, float *grad)
     grad[j] = (float)s;
 }
 
+// Adam (Kingma & Ba 2015, Algorithm 1).  ob1 = 1 - b1, ob2 = 1 - b2 and the
+// bias corrections c1, c2 come from the host in double.
 __global__ void k_adam(float *P, const float *grad, float *m, float *v, float lr, float b1, float b2,
-                       float eps, float c1, float c2)
+                       float ob1, float ob2, float eps, float c1, float c2)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= kMlpTrain) return;
     const float gj = grad[j];
-    const float mj = b1 * m[j] + (1.f - b1) * gj;
-    const float vj = b2 * v[j] + (1.f - b2) * gj * gj;
+    const float mj = b1 * m[j] + ob1 * gj;
+    const float vj = b2 * v[j] + ob2 * gj * gj;
     m[j] = mj;
     v[j] = vj;
     P[j] -= lr * (mj / c1) / (sqrtf(vj / c2) + eps);
@@ -445,12 +449,13 @@ hipError_t launch_mlp_grad(const float *P, const float *feat, const float *gnu, 
     return hipGetLastError();
 }
 
-hipError_t launch_adam(float *P, const float *grad, float *m, float *v, int step, float lr, float b1,
-                       float b2, float eps, hipStream_t s)
+hipError_t launch_adam(float *P, const float *grad, float *m, float *v, int step, double lr, double b1,
+                       double b2, double eps, hipStream_t s)
 {
-    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
-    hipLaunchKernelGGL(k_adam, dim3((kMlpTrain + 255) / 256), dim3(256), 0, s, P, grad, m, v, lr, b1, b2, eps,
-                       c1, c2);
+    const double c1 = 1.0 - std::pow(b1, step), c2 = 1.0 - std::pow(b2, step);
+    hipLaunchKernelGGL(k_adam, dim3((kMlpTrain + 255) / 256), dim3(256), 0, s, P, grad, m, v, (float)lr,
+                       (float)b1, (float)b2, (float)(1.0 - b1), (float)(1.0 - b2), (float)eps, (float)c1,
+                       (float)c2);
     return hipGetLastError();
 }
 

@@ -24,17 +24,43 @@
 // After the passes the batch's distinct nodes are written once each
 // (write_batch: plain store / read-add-store per colour, or FP64 atomics).
 //
-// T = double: the FP64 form.  T = float: the mixed form (all of the above in
-// FP32, pressure recovered in FP64 from E, rho, u when pr == nullptr,
-// node sums and assembly in FP64).
+// Storage types TG (geometry) and TF (nodal fields), FP64 arithmetic
+// throughout (msod_kernels.h).  Without a pressure array the pressure is
+// recovered as p = (gamma-1)(E - rho|u|^2/2) from the stored state.
 #include "msod_kernels.h"
 
 namespace msod {
 
-template <int P, typename T>
-__global__ void __launch_bounds__(Shape<P>::NT) k_convec(ConvecArgs<T> a)
+// Inputs of one element node, loaded one pass ahead of their use so that
+// the gathers of the next pass are in flight while this pass computes.
+template <typename TF>
+struct ConvecNode {
+    TF r, E, p, u[3], q[3];
+    uint32_t li;
+};
+
+template <int NN, typename TG, typename TF>
+__device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_t e, int l,
+                                             ConvecNode<TF> &x)
+{
+    const int64_t g = a.plan.conn[e * NN + l];
+    const int64_t np = a.npoin;
+    x.r = a.rho[g];
+    x.E = a.E[g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x.u[d] = a.u[g + d * np];
+        x.q[d] = a.q[g + d * np];
+    }
+    x.p = a.pr ? a.pr[g] : (TF)0;
+    x.li = a.plan.lidx[e * NN + l];
+}
+
+template <int P, typename TG, typename TF>
+__global__ void __launch_bounds__(Shape<P>::NT, 4) k_convec(ConvecArgs<TG, TF> a)
 {
     using S = Shape<P>;
+    using T = double;  // arithmetic
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
     constexpr int NF = 10;
     __shared__ T sD[Q * Q];
@@ -42,151 +68,144 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_convec(ConvecArgs<T> a)
     __shared__ double sR[5][UMAX];
 
     const Plan &pl = a.plan;
-    const int64_t b = pl.batches[blockIdx.x];
+    const int64_t b = pl.batches[xcd_slot(blockIdx.x, gridDim.x)];
     const uint32_t u0 = pl.uoff[b], U = pl.uoff[b + 1] - u0;
     const int t = threadIdx.x;
+    const int le = t / NN;
+    const int l = t - le * NN;
+    const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
+
+    ConvecNode<TF> cur, nxt;
+    {
+        const int64_t e = b * S::BATCH + le;
+        if (le < EPB && e < a.nelem) convec_fetch<NN>(a, e, l, cur);
+    }
     for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
     for (uint32_t i = t; i < U; i += blockDim.x)
 #pragma unroll
         for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
 
-    const int le = t / NN;
-    const int l = t - le * NN;
-    const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
-
+#pragma unroll
     for (int pass = 0; pass < S::NPASS; ++pass) {
         const int64_t e = b * S::BATCH + pass * EPB + le;
         const bool active = (le < EPB) && (e < a.nelem);
-        T r = 0, H = 0, rH = 0, uu[3] = {0, 0, 0}, qq[3] = {0, 0, 0};
         __syncthreads();  // previous pass done with sF (and sR zeroed)
         if (active) {
-            const int64_t g = pl.conn[e * NN + l];
-            const int64_t np = a.npoin;
-            r = a.rho[g];
-            const T Ev = a.E[g];
-            for (int d = 0; d < 3; ++d) {
-                uu[d] = a.u[g + d * np];
-                qq[d] = a.q[g + d * np];
-            }
-            const T k = uu[0] * uu[0] * (T)0.5 + uu[1] * uu[1] * (T)0.5 + uu[2] * uu[2] * (T)0.5;
-            T pv;
-            if (a.pr) {
-                pv = a.pr[g];
-            } else {
-                // p = (gamma-1)(E - rho|u|^2/2), evaluated in FP64
-                const double kd = 0.5 * ((double)uu[0] * uu[0] + (double)uu[1] * uu[1] +
-                                         (double)uu[2] * uu[2]);
-                pv = (T)(a.gamma_m1 * ((double)Ev - (double)r * kd));
-            }
-            H = Ev + k;
-            rH = r * Ev + r * k;
+            const T r = cur.r, Ev = cur.E;
+            const T ux = cur.u[0], uy = cur.u[1], uz = cur.u[2];
+            const T k = ux * ux * (T)0.5 + uy * uy * (T)0.5 + uz * uz * (T)0.5;
+            // p = (gamma-1)(E - rho|u|^2/2) when no pressure array is given
+            const T pv = a.pr ? (T)cur.p : a.gamma_m1 * (Ev - r * (T)0.5 * (ux * ux + uy * uy + uz * uz));
+            const T H = Ev + k;
+            const T rH = r * Ev + r * k;
             sF[le][0][l] = r;
             sF[le][1][l] = pv;
             sF[le][2][l] = H;
             sF[le][3][l] = rH;
-            sF[le][4][l] = uu[0];
-            sF[le][5][l] = uu[1];
-            sF[le][6][l] = uu[2];
-            sF[le][7][l] = qq[0];
-            sF[le][8][l] = qq[1];
-            sF[le][9][l] = qq[2];
+            sF[le][4][l] = ux;
+            sF[le][5][l] = uy;
+            sF[le][6][l] = uz;
+            sF[le][7][l] = cur.q[0];
+            sF[le][8][l] = cur.q[1];
+            sF[le][9][l] = cur.q[2];
+        }
+        if (pass + 1 < S::NPASS) {  // next pass's inputs: in flight during this pass
+            const int64_t en = e + EPB;
+            if (le < EPB && en < a.nelem) convec_fetch<NN>(a, en, l, nxt);
         }
         __syncthreads();
         double v[5] = {0, 0, 0, 0, 0};
+        const uint32_t li = cur.li;
         if (active) {
-        const T *geo = a.geo + e * 9 * NN + l;
-        // divergence-type accumulators (coefficient independent of d)
-        T aQ = 0, aU = 0, aRHU = 0, aHU = 0, aQU[3] = {0, 0, 0}, aUU[3] = {0, 0, 0};
-        // gradient-type terms, contracted per direction with the node's own values
-        T sRho = 0, sH = 0, sRH = 0, gP[3] = {0, 0, 0}, qGu[3] = {0, 0, 0}, uGq[3] = {0, 0, 0};
-
+            // the node's own values are read back from LDS where needed, so
+            // that registers stay free for the next pass's loads
+            const TG *geo = a.geo + e * 9 * NN + l;
+            // divergence-type accumulators (coefficient independent of d)
+            T aQ = 0, aU = 0, aRHU = 0, aHU = 0, aQU[3] = {0, 0, 0}, aUU[3] = {0, 0, 0};
+            // gradient-type terms, contracted per direction with the node's own values
+            T sRho = 0, sH = 0, sRH = 0, gP[3] = {0, 0, 0}, qGu[3] = {0, 0, 0}, uGq[3] = {0, 0, 0};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const T h0 = geo[(0 + 3 * d) * NN], h1 = geo[(1 + 3 * d) * NN], h2 = geo[(2 + 3 * d) * NN];
-            const T Ua = h0 * uu[0] + h1 * uu[1] + h2 * uu[2];
-            const T Qa = h0 * qq[0] + h1 * qq[1] + h2 * qq[2];
-            const int Id = d == 0 ? I : (d == 1 ? J : K);
-            const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
-            const int base = l - Id * stride;
-            T aRho = 0, aP = 0, aH = 0, aRH = 0, aUc[3] = {0, 0, 0}, aQc[3] = {0, 0, 0};
+            for (int d = 0; d < 3; ++d) {
+                const T h0 = geo[(0 + 3 * d) * NN], h1 = geo[(1 + 3 * d) * NN], h2 = geo[(2 + 3 * d) * NN];
+                const T Ua = h0 * sF[le][4][l] + h1 * sF[le][5][l] + h2 * sF[le][6][l];
+                const T Qa = h0 * sF[le][7][l] + h1 * sF[le][8][l] + h2 * sF[le][9][l];
+                const int Id = d == 0 ? I : (d == 1 ? J : K);
+                const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
+                const int base = l - Id * stride;
 #pragma unroll 1
-            for (int n = 0; n < Q; ++n) {
-                const int s = base + n * stride;
-                const T Dv = sD[Id * Q + n];
-                const T rs = sF[le][0][s], ps = sF[le][1][s], Hs = sF[le][2][s], rHs = sF[le][3][s];
-                const T us0 = sF[le][4][s], us1 = sF[le][5][s], us2 = sF[le][6][s];
-                const T qs0 = sF[le][7][s], qs1 = sF[le][8][s], qs2 = sF[le][9][s];
-                const T Us = h0 * us0 + h1 * us1 + h2 * us2;
-                const T Qs = h0 * qs0 + h1 * qs1 + h2 * qs2;
-                aRho += Dv * rs;
-                aP += Dv * ps;
-                aH += Dv * Hs;
-                aRH += Dv * rHs;
-                aUc[0] += Dv * us0; aUc[1] += Dv * us1; aUc[2] += Dv * us2;
-                aQc[0] += Dv * qs0; aQc[1] += Dv * qs1; aQc[2] += Dv * qs2;
-                aU += Dv * Us;
-                aQ += Dv * Qs;
-                const T DU = Dv * Us;
-                aRHU += DU * rHs;
-                aHU += DU * Hs;
-                aQU[0] += DU * qs0; aQU[1] += DU * qs1; aQU[2] += DU * qs2;
-                aUU[0] += DU * us0; aUU[1] += DU * us1; aUU[2] += DU * us2;
+                for (int n = 0; n < Q; ++n) {
+                    const int s = base + n * stride;
+                    const T Dv = sD[Id * Q + n];
+                    const T rs = sF[le][0][s], ps = sF[le][1][s], Hs = sF[le][2][s], rHs = sF[le][3][s];
+                    const T us0 = sF[le][4][s], us1 = sF[le][5][s], us2 = sF[le][6][s];
+                    const T qs0 = sF[le][7][s], qs1 = sF[le][8][s], qs2 = sF[le][9][s];
+                    // the node's own u, q, He column fold into the weights of this source
+                    const T UaD = Ua * Dv, QaD = Qa * Dv;
+                    sRho += UaD * rs;
+                    sRH += UaD * rHs;
+                    sH += QaD * Hs;
+                    gP[0] += (h0 * Dv) * ps; gP[1] += (h1 * Dv) * ps; gP[2] += (h2 * Dv) * ps;
+                    qGu[0] += QaD * us0; qGu[1] += QaD * us1; qGu[2] += QaD * us2;
+                    uGq[0] += UaD * qs0; uGq[1] += UaD * qs1; uGq[2] += UaD * qs2;
+                    const T DU = Dv * (h0 * us0 + h1 * us1 + h2 * us2);
+                    const T DQ = Dv * (h0 * qs0 + h1 * qs1 + h2 * qs2);
+                    aU += DU;
+                    aQ += DQ;
+                    aRHU += DU * rHs;
+                    aHU += DU * Hs;
+                    aQU[0] += DU * qs0; aQU[1] += DU * qs1; aQU[2] += DU * qs2;
+                    aUU[0] += DU * us0; aUU[1] += DU * us1; aUU[2] += DU * us2;
+                }
             }
-            sRho += Ua * aRho;
-            sH += Qa * aH;
-            sRH += Ua * aRH;
-            gP[0] += h0 * aP; gP[1] += h1 * aP; gP[2] += h2 * aP;
+            const T gv = a.gvol[e * NN + l];
+            const T r = sF[le][0][l], H = sF[le][2][l], rH = sF[le][3][l];
+            const T uu[3] = {sF[le][4][l], sF[le][5][l], sF[le][6][l]};
+            const T qq[3] = {sF[le][7][l], sF[le][8][l], sF[le][9][l]};
+            const T Rm = (T)0.5 * (aQ + r * aU) + (T)0.5 * sRho;
+            const T Re = (T)0.25 * (aRHU + H * aQ + rH * aU + r * aHU) + (T)0.25 * (sH + sRH + H * sRho);
+            T Rmo[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                qGu[c] += Qa * aUc[c];
-                uGq[c] += Ua * aQc[c];
-            }
+            for (int i = 0; i < 3; ++i)
+                Rmo[i] = (T)0.25 * (aQU[i] + uu[i] * aQ + qq[i] * aU + r * aUU[i] + uu[i] * sRho +
+                                    qGu[i] + uGq[i]) + gP[i];
+            v[0] = (double)(gv * Rm);
+            v[1] = (double)(gv * Rmo[0]);
+            v[2] = (double)(gv * Rmo[1]);
+            v[3] = (double)(gv * Rmo[2]);
+            v[4] = (double)(gv * Re);
         }
-
-        const T gv = a.gvol[e * NN + l];
-        const T Rm = (T)0.5 * (aQ + r * aU) + (T)0.5 * sRho;
-        const T Re = (T)0.25 * (aRHU + H * aQ + rH * aU + r * aHU) + (T)0.25 * (sH + sRH + H * sRho);
-        T Rmo[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-            Rmo[i] = (T)0.25 * (aQU[i] + uu[i] * aQ + qq[i] * aU + r * aUU[i] + uu[i] * sRho +
-                                qGu[i] + uGq[i]) + gP[i];
-        v[0] = (double)(gv * Rm);
-        v[1] = (double)(gv * Rmo[0]);
-        v[2] = (double)(gv * Rmo[1]);
-        v[3] = (double)(gv * Rmo[2]);
-        v[4] = (double)(gv * Re);
-        }
-        add_pass<UMAX, EPB>(sR, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
+        add_pass<UMAX, EPB>(sR, le, active, active ? li : 0u, v);
+        if (pass + 1 < S::NPASS) cur = nxt;
     }
     write_batch<UMAX>(pl, u0, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 
-template <int P, typename T>
-static hipError_t launch_p(const ConvecArgs<T> &a, hipStream_t s)
+template <int P, typename TG, typename TF>
+static hipError_t launch_p(const ConvecArgs<TG, TF> &a, hipStream_t s)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_convec<P, T>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s, a);
+    hipLaunchKernelGGL((k_convec<P, TG, TF>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s, a);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_convec(int porder, const ConvecArgs<T> &a, hipStream_t s)
+template <typename TG, typename TF>
+hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, T>(a, s);
-    case 2: return launch_p<2, T>(a, s);
-    case 3: return launch_p<3, T>(a, s);
-    case 4: return launch_p<4, T>(a, s);
-    case 5: return launch_p<5, T>(a, s);
-    case 6: return launch_p<6, T>(a, s);
-    case 7: return launch_p<7, T>(a, s);
-    case 8: return launch_p<8, T>(a, s);
+    case 1: return launch_p<1, TG, TF>(a, s);
+    case 2: return launch_p<2, TG, TF>(a, s);
+    case 3: return launch_p<3, TG, TF>(a, s);
+    case 4: return launch_p<4, TG, TF>(a, s);
+    case 5: return launch_p<5, TG, TF>(a, s);
+    case 6: return launch_p<6, TG, TF>(a, s);
+    case 7: return launch_p<7, TG, TF>(a, s);
+    case 8: return launch_p<8, TG, TF>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_convec<double>(int, const ConvecArgs<double> &, hipStream_t);
-template hipError_t launch_convec<float>(int, const ConvecArgs<float> &, hipStream_t);
+template hipError_t launch_convec<double, double>(int, const ConvecArgs<double, double> &, hipStream_t);
+template hipError_t launch_convec<float, float>(int, const ConvecArgs<float, float> &, hipStream_t);
+template hipError_t launch_convec<float, double>(int, const ConvecArgs<float, double> &, hipStream_t);
 
 }  // namespace msod

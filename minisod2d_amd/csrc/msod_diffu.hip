@@ -13,7 +13,8 @@
 //        F_d = gpvol * sum_i He(i,d) Phi_i        (6 values per node)
 //    and the target accumulates the transposed sweep
 //        W[Phi](a) += sum_n D[n][I_d(a)] F_d(s_d(n))
-//    (LDS holds 6 values per node instead of 18).  Then
+//    (LDS holds 6 values per node instead of 18, in the buffer pass 1 used
+//    for rho, T, u).  Then
 //    Rmass += nu_e W[grad rho], Rener += W[tau.u] + kappa_e W[grad T],
 //    Rmom_r += W[tau_r], with nu_e and kappa_e of the target node (the
 //    reference multiplies kappa_e inside the sum; same value, other rounding).
@@ -21,8 +22,9 @@
 // in LDS in a fixed order (add_pass); the batch's distinct nodes are then
 // written once (write_batch).
 //
-// T = double: FP64 form; T = float: mixed form (FP32 sweeps, FP64 node sums
-// and assembly).  CLOSURE (FP64 only): mu_sgs is not read but predicted per
+// Storage types TG (geometry, mu_e, mu_sgs) and TF (nodal fields), FP64
+// arithmetic throughout (msod_kernels.h).  CLOSURE (FP64 storage only):
+// mu_sgs is not read but predicted per
 // element node by the learned closure (msod_mlp.h) from the pass-1 velocity
 // gradient, evaluated with bf16 MFMA inside the kernel before the stress is
 // formed; its LDS scratch overlays the field buffers between the passes.
@@ -34,55 +36,90 @@
 
 namespace msod {
 
-template <int P, typename T, bool CLOSURE>
-__global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a, const float *mlp)
+// Inputs of one element node, loaded one pass ahead of their use so that
+// the gathers of the next pass are in flight while this pass computes.
+template <typename TG, typename TF>
+struct DiffuNode {
+    TF r, T_, muf, u[3];
+    TG mue, mus;
+};
+
+template <int NN, bool CLOSURE, typename TG, typename TF>
+__device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t e, int l,
+                                            DiffuNode<TG, TF> &x)
+{
+    const int64_t g = a.plan.conn[e * NN + l];
+    const int64_t np = a.npoin;
+    x.r = a.rho[g];
+    x.T_ = a.Tem[g];
+    x.muf = a.mu_fluid[g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x.u[d] = a.u[g + d * np];
+    const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
+    x.mue = a.mu_e[ge];
+    x.mus = (!CLOSURE && a.mu_sgs) ? a.mu_sgs[ge] : (TG)0;  // NULL: closure fallback
+}
+
+template <int P, typename TG, typename TF, bool CLOSURE>
+__global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, const float *mlp)
 {
     using S = Shape<P>;
+    using T = double;  // arithmetic
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
+    // sA (pass 1: rho, T, u) and sG (pass 2: fluxes of one direction) share
+    // one buffer; so does the MLP scratch of the closure between the two
     constexpr size_t kSA = sizeof(T) * EPB * 5 * NN, kSG = sizeof(T) * EPB * 6 * NN;
     constexpr size_t kScr = CLOSURE ? (size_t)(S::NT / 64) * 64 * kMlpH * sizeof(short) : 0;
-    constexpr size_t kBuf = kSA + kSG > kScr ? kSA + kSG : kScr;
+    constexpr size_t kAG = kSA > kSG ? kSA : kSG;
+    constexpr size_t kBuf = kAG > kScr ? kAG : kScr;
     __shared__ T sD[Q * Q];
     __shared__ __attribute__((aligned(16))) unsigned char sbuf[kBuf];
     __shared__ double sR[5][UMAX];
     __shared__ typename std::conditional<CLOSURE, MlpShared, char>::type smlp;
-    auto sA = reinterpret_cast<T (*)[5][NN]>(sbuf);        // rho, T, u0, u1, u2
-    auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf + kSA);  // contravariant fluxes of one direction
+    auto sA = reinterpret_cast<T (*)[5][NN]>(sbuf);  // rho, T, u0, u1, u2
+    auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf);  // contravariant fluxes of one direction
 
     const Plan &pl = a.plan;
-    const int64_t b = pl.batches[blockIdx.x];
+    const int64_t b = pl.batches[xcd_slot(blockIdx.x, gridDim.x)];
     const uint32_t u0 = pl.uoff[b], U = pl.uoff[b + 1] - u0;
     const int t = threadIdx.x;
+    const int le = t / NN;
+    const int l = t - le * NN;
+    const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
+    const T Cp = (T)a.Cp, Pr = (T)a.Pr;
+
+    DiffuNode<TG, TF> cur, nxt;
+    {
+        const int64_t e = b * S::BATCH + le;
+        if (le < EPB && e < a.nelem) diffu_fetch<NN, CLOSURE>(a, e, l, cur);
+    }
     for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
     for (uint32_t i = t; i < U; i += blockDim.x)
 #pragma unroll
         for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
     if constexpr (CLOSURE) mlp_stage(smlp, mlp);
 
-    const int le = t / NN;
-    const int l = t - le * NN;
-    const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
-    const T Cp = (T)a.Cp, Pr = (T)a.Pr;
-
+#pragma unroll
     for (int pass = 0; pass < S::NPASS; ++pass) {
         const int64_t e = b * S::BATCH + pass * EPB + le;
         const bool active = (le < EPB) && (e < a.nelem);
         T r = 0, muf = 0, mue = 0, mus = 0, uu[3] = {0, 0, 0};
         __syncthreads();  // previous pass done with sA/sG (and sR zeroed)
         if (active) {
-            const int64_t g = pl.conn[e * NN + l];
-            const int64_t np = a.npoin;
-            r = a.rho[g];
-            muf = a.mu_fluid[g];
-            const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
-            mue = a.mu_e[ge];
-            if constexpr (!CLOSURE) mus = a.mu_sgs ? a.mu_sgs[ge] : (T)0;  // NULL: closure fallback
+            r = cur.r;
+            muf = cur.muf;
+            mue = cur.mue;
+            mus = cur.mus;
             sA[le][0][l] = r;
-            sA[le][1][l] = a.Tem[g];
+            sA[le][1][l] = cur.T_;
             for (int d = 0; d < 3; ++d) {
-                uu[d] = a.u[g + d * np];
+                uu[d] = cur.u[d];
                 sA[le][2 + d][l] = uu[d];
             }
+        }
+        if (pass + 1 < S::NPASS) {  // next pass's inputs: in flight during this pass
+            const int64_t en = e + EPB;
+            if (le < EPB && en < a.nelem) diffu_fetch<NN, CLOSURE>(a, en, l, nxt);
         }
         __syncthreads();
 
@@ -91,7 +128,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a, const fl
         T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         T tauU[3] = {0, 0, 0}, gv = 0, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (active) {
-            const T *geo = a.geo + e * 9 * NN + l;
+            const TG *geo = a.geo + e * 9 * NN + l;
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -154,6 +191,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a, const fl
         }
 
         // ---- pass 2: transposed sweeps, one reference direction at a time -
+        __syncthreads();  // every sweep over sA done: sG overlays it
         T W[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -192,50 +230,54 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<T> a, const fl
             v[4] = (double)(W[1] + kappa * W[2]);
         }
         add_pass<UMAX, EPB>(sR, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
+        if (pass + 1 < S::NPASS) cur = nxt;
     }
     write_batch<UMAX>(pl, u0, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 
-template <int P, typename T, bool CLOSURE = false>
-static hipError_t launch_p(const DiffuArgs<T> &a, hipStream_t s, const float *mlp = nullptr)
+template <int P, typename TG, typename TF, bool CLOSURE = false>
+static hipError_t launch_p(const DiffuArgs<TG, TF> &a, hipStream_t s, const float *mlp = nullptr)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_diffu<P, T, CLOSURE>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s, a, mlp);
+    hipLaunchKernelGGL((k_diffu<P, TG, TF, CLOSURE>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s,
+                       a, mlp);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_diffu(int porder, const DiffuArgs<T> &a, hipStream_t s)
+template <typename TG, typename TF>
+hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, T>(a, s);
-    case 2: return launch_p<2, T>(a, s);
-    case 3: return launch_p<3, T>(a, s);
-    case 4: return launch_p<4, T>(a, s);
-    case 5: return launch_p<5, T>(a, s);
-    case 6: return launch_p<6, T>(a, s);
-    case 7: return launch_p<7, T>(a, s);
-    case 8: return launch_p<8, T>(a, s);
+    case 1: return launch_p<1, TG, TF>(a, s);
+    case 2: return launch_p<2, TG, TF>(a, s);
+    case 3: return launch_p<3, TG, TF>(a, s);
+    case 4: return launch_p<4, TG, TF>(a, s);
+    case 5: return launch_p<5, TG, TF>(a, s);
+    case 6: return launch_p<6, TG, TF>(a, s);
+    case 7: return launch_p<7, TG, TF>(a, s);
+    case 8: return launch_p<8, TG, TF>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_diffu<double>(int, const DiffuArgs<double> &, hipStream_t);
+template hipError_t launch_diffu<double, double>(int, const DiffuArgs<double, double> &, hipStream_t);
+template hipError_t launch_diffu<float, float>(int, const DiffuArgs<float, float> &, hipStream_t);
+template hipError_t launch_diffu<float, double>(int, const DiffuArgs<float, double> &, hipStream_t);
 
-hipError_t launch_diffu_closure(int porder, const DiffuArgs<double> &a, const float *mlp, hipStream_t s)
+hipError_t launch_diffu_closure(int porder, const DiffuArgs<double, double> &a, const float *mlp,
+                                hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, double, true>(a, s, mlp);
-    case 2: return launch_p<2, double, true>(a, s, mlp);
-    case 3: return launch_p<3, double, true>(a, s, mlp);
-    case 4: return launch_p<4, double, true>(a, s, mlp);
-    case 5: return launch_p<5, double, true>(a, s, mlp);
-    case 6: return launch_p<6, double, true>(a, s, mlp);
-    case 7: return launch_p<7, double, true>(a, s, mlp);
-    case 8: return launch_p<8, double, true>(a, s, mlp);
+    case 1: return launch_p<1, double, double, true>(a, s, mlp);
+    case 2: return launch_p<2, double, double, true>(a, s, mlp);
+    case 3: return launch_p<3, double, double, true>(a, s, mlp);
+    case 4: return launch_p<4, double, double, true>(a, s, mlp);
+    case 5: return launch_p<5, double, double, true>(a, s, mlp);
+    case 6: return launch_p<6, double, double, true>(a, s, mlp);
+    case 7: return launch_p<7, double, double, true>(a, s, mlp);
+    case 8: return launch_p<8, double, double, true>(a, s, mlp);
     default: return hipErrorInvalidValue;
     }
 }
-template hipError_t launch_diffu<float>(int, const DiffuArgs<float> &, hipStream_t);
 
 }  // namespace msod

@@ -70,34 +70,43 @@ struct Plan {
     int mode;
 };
 
-template <typename T>
+// Kernel arguments.  Storage types: TG for the element-sized arrays (geo,
+// gvol; in the diffusion also mu_e, mu_sgs), TF for the nodal fields.  The
+// kernels compute in FP64 whatever the storage: on gfx950 an FP64 FMA issues
+// at the rate of an unpacked FP32 one, so FP32 storage buys bytes, not time
+// in the ALUs, and FP64 arithmetic keeps rounding out of the derivative
+// sweeps (whose cancellation would amplify FP32 rounding by about 1/h).
+//   <double, double>  FP64 form (the reference's operands)
+//   <float,  float>   FP32 storage of every input
+//   <float,  double>  FP32 element arrays, FP64 nodal fields
+template <typename TG, typename TF>
 struct ConvecArgs {
     int64_t nelem, npoin;
     Plan plan;
-    const T *geo, *gvol;
-    const T *u, *q, *rho, *pr, *E;   // pr == nullptr: recover pressure from E
+    const TG *geo, *gvol;
+    const TF *u, *q, *rho, *pr, *E;  // pr == nullptr: recover pressure from E
     double gamma_m1;                 // gamma-1, used when pr == nullptr
     double *Rmass, *Rmom, *Rener;
     Tables tab;
 };
 
-template <typename T>
+template <typename TG, typename TF>
 struct DiffuArgs {
     int64_t nelem, npoin;
     Plan plan;
-    const T *geo, *gvol;
-    const T *rho, *u, *Tem, *mu_fluid, *mu_e, *mu_sgs;
+    const TG *geo, *gvol;
+    const TF *rho, *u, *Tem, *mu_fluid;
+    const TG *mu_e, *mu_sgs;
     double Cp, Pr, c_rho, c_ener;
     double *Rmass, *Rmom, *Rener;
     Tables tab;
 };
 
-// Launchers (return hipError_t of the launch).  T = double: FP64 form;
-// T = float: mixed form (FP32 sweeps, FP64 assembly).
-template <typename T>
-hipError_t launch_convec(int porder, const ConvecArgs<T> &a, hipStream_t s);
-template <typename T>
-hipError_t launch_diffu(int porder, const DiffuArgs<T> &a, hipStream_t s);
+// Launchers (return hipError_t of the launch).
+template <typename TG, typename TF>
+hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s);
+template <typename TG, typename TF>
+hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s);
 
 // Zero Rmass (n), Rmom (3n), Rener (n).
 hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64_t n,
@@ -110,6 +119,18 @@ hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t
 template <typename T>
 hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
                             const int32_t *l2g, T *geo, T *gvol, hipStream_t s);
+
+// Blocks of a launch are dealt to the chip's 8 XCDs round robin (block i to
+// XCD i % 8), each XCD with its own L2.  xcd_slot gives every XCD a
+// contiguous range of the launch's batch list, so that batches adjacent in
+// the list (which share nodes and cache lines) meet in the same L2.
+// Bijective on [0, n) for any n.
+__device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n)
+{
+    constexpr uint32_t kXcd = 8;
+    const uint32_t x = i % kXcd, k = i / kXcd, q = n / kXcd, r = n % kXcd;
+    return x * q + (x < r ? x : r) + k;
+}
 
 // Write phase shared by the kernels: the batch's node sums sR[c][i]
 // (c = Rmass, Rmom x3, Rener) go to the outputs.

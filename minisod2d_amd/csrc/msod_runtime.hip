@@ -177,7 +177,7 @@ struct DerivedGeo {
     void *geo, *gvol;
 };
 
-constexpr int kNumTimers = 7;
+constexpr int kNumTimers = 9;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -195,7 +195,7 @@ struct State {
     std::vector<TableSet *> tables;
     std::vector<PlanEntry> plans;
     std::vector<DerivedGeo> geos;
-    int assembly = kColored;
+    int assembly = kAtomic;
     double ms[kNumTimers] = {0};
     int64_t launches[kNumTimers] = {0};
     std::vector<Pending> pending;
@@ -592,12 +592,12 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     return MSOD_OK;
 }
 
-// Shared front half of the convective calls.
-template <typename T>
-int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const T *He,
-                  const T *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
-                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, const T *u,
-                  const T *q, const T *rho, const T *pr, const T *E, double gamma_gas,
+// Shared front half of the convective calls (TG: geometry, TF: fields).
+template <typename TG, typename TF>
+int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const TG *He,
+                  const TG *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, const TF *u,
+                  const TF *q, const TF *rho, const TF *pr, const TF *E, double gamma_gas,
                   double *Rmass, double *Rmom, double *Rener, int timer)
 {
     NEED_INIT();
@@ -610,49 +610,49 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     const void *dc, *dHe, *dV, *du, *dq, *dr, *dp = nullptr, *dE, *dRm, *dRmo, *dRe;
     uint64_t vc, vHe, vV, vx;
     if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
-    if ((st = resolve(He, sizeof(T) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
-    if ((st = resolve(gpvol, sizeof(T) * NE * nn, "gpvol", &dV, &vV))) return st;
-    if ((st = resolve(u, sizeof(T) * 3 * NP, "u", &du, &vx))) return st;
-    if ((st = resolve(q, sizeof(T) * 3 * NP, "q", &dq, &vx))) return st;
-    if ((st = resolve(rho, sizeof(T) * NP, "rho", &dr, &vx))) return st;
-    if (pr && (st = resolve(pr, sizeof(T) * NP, "pr", &dp, &vx))) return st;
-    if ((st = resolve(E, sizeof(T) * NP, "E", &dE, &vx))) return st;
+    if ((st = resolve(He, sizeof(TG) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
+    if ((st = resolve(gpvol, sizeof(TG) * NE * nn, "gpvol", &dV, &vV))) return st;
+    if ((st = resolve(u, sizeof(TF) * 3 * NP, "u", &du, &vx))) return st;
+    if ((st = resolve(q, sizeof(TF) * 3 * NP, "q", &dq, &vx))) return st;
+    if ((st = resolve(rho, sizeof(TF) * NP, "rho", &dr, &vx))) return st;
+    if (pr && (st = resolve(pr, sizeof(TF) * NP, "pr", &dp, &vx))) return st;
+    if ((st = resolve(E, sizeof(TF) * NP, "E", &dE, &vx))) return st;
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
-    const T *geo, *gvol;
+    const TG *geo, *gvol;
     if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
-    if ((st = get_geo<T>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+    if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
-    ConvecArgs<T> a;
+    ConvecArgs<TG, TF> a;
     a.nelem = NE;
     a.npoin = NP;
     a.geo = geo;
     a.gvol = gvol;
-    a.u = (const T *)du;
-    a.q = (const T *)dq;
-    a.rho = (const T *)dr;
-    a.pr = (const T *)dp;
-    a.E = (const T *)dE;
+    a.u = (const TF *)du;
+    a.q = (const TF *)dq;
+    a.rho = (const TF *)dr;
+    a.pr = (const TF *)dp;
+    a.E = (const TF *)dE;
     a.gamma_m1 = gamma_gas - 1.0;
     a.Rmass = (double *)dRm;
     a.Rmom = (double *)dRmo;
     a.Rener = (double *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
     if ((st = run_assembled(a, *plan, timer,
-                            [&](const ConvecArgs<T> &x) { return launch_convec<T>(porder, x, g.stream); },
+                            [&](const ConvecArgs<TG, TF> &x) { return launch_convec<TG, TF>(porder, x, g.stream); },
                             "convection kernel launch")))
         return st;
     return finish_call("convection kernel");
 }
 
-template <typename T>
-int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const T *He,
-                 const T *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+template <typename TG, typename TF>
+int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const TG *He,
+                 const TG *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, double Cp,
-                 double Pr, const T *rho, const T *u, const T *Tem, const T *mu_fluid,
-                 const T *mu_e, const T *mu_sgs, double *Rmass, double *Rmom, double *Rener,
+                 double Pr, const TF *rho, const TF *u, const TF *Tem, const TF *mu_fluid,
+                 const TG *mu_e, const TG *mu_sgs, double *Rmass, double *Rmom, double *Rener,
                  int timer, bool closure = false, const float *dmlp = nullptr)
 {
     NEED_INIT();
@@ -665,33 +665,33 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     const void *dc, *dHe, *dV, *dr, *du, *dT, *dmf, *dme, *dms = nullptr, *dRm, *dRmo, *dRe;
     uint64_t vc, vHe, vV, vx;
     if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
-    if ((st = resolve(He, sizeof(T) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
-    if ((st = resolve(gpvol, sizeof(T) * NE * nn, "gpvol", &dV, &vV))) return st;
-    if ((st = resolve(rho, sizeof(T) * NP, "rho", &dr, &vx))) return st;
-    if ((st = resolve(u, sizeof(T) * 3 * NP, "u", &du, &vx))) return st;
-    if ((st = resolve(Tem, sizeof(T) * NP, "Tem", &dT, &vx))) return st;
-    if ((st = resolve(mu_fluid, sizeof(T) * NP, "mu_fluid", &dmf, &vx))) return st;
-    if ((st = resolve(mu_e, sizeof(T) * NE * nn, "mu_e", &dme, &vx))) return st;
-    if (!closure && (st = resolve(mu_sgs, sizeof(T) * NE * nn, "mu_sgs", &dms, &vx))) return st;
+    if ((st = resolve(He, sizeof(TG) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
+    if ((st = resolve(gpvol, sizeof(TG) * NE * nn, "gpvol", &dV, &vV))) return st;
+    if ((st = resolve(rho, sizeof(TF) * NP, "rho", &dr, &vx))) return st;
+    if ((st = resolve(u, sizeof(TF) * 3 * NP, "u", &du, &vx))) return st;
+    if ((st = resolve(Tem, sizeof(TF) * NP, "Tem", &dT, &vx))) return st;
+    if ((st = resolve(mu_fluid, sizeof(TF) * NP, "mu_fluid", &dmf, &vx))) return st;
+    if ((st = resolve(mu_e, sizeof(TG) * NE * nn, "mu_e", &dme, &vx))) return st;
+    if (!closure && (st = resolve(mu_sgs, sizeof(TG) * NE * nn, "mu_sgs", &dms, &vx))) return st;
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
-    const T *geo, *gvol;
+    const TG *geo, *gvol;
     if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
-    if ((st = get_geo<T>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+    if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
-    DiffuArgs<T> a;
+    DiffuArgs<TG, TF> a;
     a.nelem = NE;
     a.npoin = NP;
     a.geo = geo;
     a.gvol = gvol;
-    a.rho = (const T *)dr;
-    a.u = (const T *)du;
-    a.Tem = (const T *)dT;
-    a.mu_fluid = (const T *)dmf;
-    a.mu_e = (const T *)dme;
-    a.mu_sgs = (const T *)dms;
+    a.rho = (const TF *)dr;
+    a.u = (const TF *)du;
+    a.Tem = (const TF *)dT;
+    a.mu_fluid = (const TF *)dmf;
+    a.mu_e = (const TG *)dme;
+    a.mu_sgs = (const TG *)dms;
     a.Cp = Cp;
     a.Pr = Pr;
     a.c_rho = g.c_rho;
@@ -701,10 +701,10 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     a.Rener = (double *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
     if ((st = run_assembled(a, *plan, timer,
-                            [&](const DiffuArgs<T> &x) {
-                                if constexpr (std::is_same<T, double>::value)
+                            [&](const DiffuArgs<TG, TF> &x) {
+                                if constexpr (std::is_same<TG, double>::value && std::is_same<TF, double>::value)
                                     if (dmlp) return launch_diffu_closure(porder, x, dmlp, g.stream);
-                                return launch_diffu<T>(porder, x, g.stream);
+                                return launch_diffu<TG, TF>(porder, x, g.stream);
                             },
                             "diffusion kernel launch")))
         return st;
@@ -1082,7 +1082,7 @@ int msod_full_convec_ijk(int porder, int nelem, int npoin, const int32_t *connec
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     (void)Ngp; (void)dNgp; (void)xgp;
     if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
-    return convec_common<double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+    return convec_common<double, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                  gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, 0.0, Rmass, Rmom,
                                  Rener, 1);
 }
@@ -1096,7 +1096,7 @@ int msod_full_convec_ijk_mixed(int porder, int nelem, int npoin, const int32_t *
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
-    return convec_common<float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+    return convec_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
                                 Rmass, Rmom, Rener, 3);
 }
@@ -1112,7 +1112,7 @@ int msod_full_diffusion_ijk(int porder, int nelem, int npoin, const int32_t *con
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     (void)Ngp; (void)dNgp; (void)xgp;
-    return diffu_common<double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+    return diffu_common<double, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                 mu_sgs, Rmass, Rmom, Rener, 2);
 }
@@ -1126,9 +1126,37 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
                                   double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    return diffu_common<float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+    return diffu_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                mu_sgs, Rmass, Rmom, Rener, 4);
+}
+
+int msod_full_convec_ijk_geo32(int porder, int nelem, int npoin, const int32_t *connec,
+                               const float *He, const float *gpvol, const double *dlxigp_ip,
+                               const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                               const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const double *u,
+                               const double *q, const double *rho, const double *pr,
+                               const double *E, double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
+    return convec_common<float, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                        gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, 0.0, Rmass,
+                                        Rmom, Rener, 7);
+}
+
+int msod_full_diffusion_ijk_geo32(int porder, int nelem, int npoin, const int32_t *connec,
+                                  const float *He, const float *gpvol, const double *dlxigp_ip,
+                                  const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                                  const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                                  double Pr, const double *rho, const double *u, const double *Tem,
+                                  const double *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                  double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    return diffu_common<float, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                       gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
+                                       mu_sgs, Rmass, Rmom, Rener, 8);
 }
 
 int msod_full_diffusion_ijk_closure(int porder, int nelem, int npoin, const int32_t *connec,
@@ -1152,7 +1180,7 @@ int msod_full_diffusion_ijk_closure(int porder, int nelem, int npoin, const int3
     if (st) return st;
     if (used_model) *used_model = usable ? 1 : 0;
     // unusable model: mu_sgs = 0 (the plain kernel with no mu_sgs array)
-    return diffu_common<double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+    return diffu_common<double, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                 nullptr, Rmass, Rmom, Rener, 5, true, usable ? dmlp : nullptr);
 }
@@ -1274,8 +1302,8 @@ int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, 
     if ((st = resolve(m, sizeof(float) * kMlpTrain, "m", &dm, &vx))) return st;
     if ((st = resolve(v, sizeof(float) * kMlpTrain, "v", &dv, &vx))) return st;
     TimedLaunch tk(6);
-    HIPCHK(launch_adam((float *)dp, (const float *)dg, (float *)dm, (float *)dv, step, (float)lr,
-                       (float)beta1, (float)beta2, (float)eps, g.stream),
+    HIPCHK(launch_adam((float *)dp, (const float *)dg, (float *)dm, (float *)dv, step, lr, beta1, beta2,
+                       eps, g.stream),
            "adam launch");
     tk.done();
     return finish_call("adam");

@@ -65,10 +65,11 @@ end module elem_convec
 module elem_diffu
 
    use iso_c_binding
-   use msod_device, only: msod_full_diffusion_ijk, msod_set_diffusion_coefficients, msod_check
+   use msod_device, only: msod_full_diffusion_ijk, msod_set_diffusion_coefficients, msod_check, &
+                          msod_full_diffusion_ijk_closure
    implicit none
    private
-   public :: full_diffusion_ijk, c_rho, c_ener
+   public :: full_diffusion_ijk, full_diffusion_ijk_closure, c_rho, c_ener
 
    real(8) :: c_rho = 1.0d0
    real(8) :: c_ener = 1.0d0
@@ -101,5 +102,40 @@ contains
                       gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, &
                       mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener), 'full_diffusion_ijk')
    end subroutine full_diffusion_ijk
+
+   ! full_diffusion_ijk with mu_sgs predicted by the learned closure inside the
+   ! kernel (no counterpart in the reference).  mlp_params: the 1530 real(4)
+   ! weights, present on the device or host memory.  used_model = .false. when
+   ! the weights are unusable (non-finite, negative scale): mu_sgs = 0 then.
+   subroutine full_diffusion_ijk_closure(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, &
+                                         invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, &
+                                         mu_fluid, mu_e, mlp_params, used_model, Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
+      real(8), intent(in), contiguous :: gpvol(:,:,:)
+      integer(4), intent(in), contiguous :: invAtoIJK(:,:,:), gmshAtoI(:), gmshAtoJ(:), gmshAtoK(:)
+      real(8), intent(in) :: Cp, Pr
+      real(8), intent(in), contiguous :: rho(:), u(:,:), Tem(:), mu_fluid(:), mu_e(:,:)
+      real(4), intent(in), contiguous :: mlp_params(:)
+      logical, intent(out) :: used_model
+      real(8), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      integer :: porder, nn
+      integer(c_int) :: used
+      porder = size(invAtoIJK, 1) - 1
+      nn = (porder+1)**3
+      if (size(connec,1) /= nelem .or. size(connec,2) /= nn .or. size(He,4) < nelem .or. &
+          size(mu_e,1) /= nelem .or. size(mu_e,2) /= nn .or. size(mlp_params) < 1530 .or. &
+          size(u,1) < npoin .or. size(Rmass) < npoin .or. size(Rmom,1) < npoin) then
+         write(*,'(a,i0,a,i0,a,i0)') 'msod: full_diffusion_ijk_closure: array shapes do not match nelem=', &
+              nelem, ' npoin=', npoin, ' porder=', porder
+         error stop 2
+      end if
+      call msod_check(msod_set_diffusion_coefficients(c_rho, c_ener), 'set_diffusion_coefficients')
+      call msod_check(msod_full_diffusion_ijk_closure(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, &
+                      He, gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, &
+                      Tem, mu_fluid, mu_e, mlp_params, used, Rmass, Rmom, Rener), 'full_diffusion_ijk_closure')
+      used_model = used /= 0
+   end subroutine full_diffusion_ijk_closure
 
 end module elem_diffu

@@ -19,6 +19,7 @@ module msod_device
    public :: msod_delete, msod_sync, msod_check, msod_set_async_mode, msod_set_assembly_mode
    public :: msod_full_convec_ijk, msod_full_diffusion_ijk, msod_full_convec_ijk_mixed
    public :: msod_full_diffusion_ijk_mixed, msod_set_diffusion_coefficients
+   public :: msod_full_diffusion_ijk_closure
 
    interface
       integer(c_int) function msod_init(device) bind(C, name='msod_init')
@@ -115,6 +116,19 @@ module msod_device
          type(*), dimension(*), intent(in) :: rho, u, Tem, mu_fluid, mu_e, mu_sgs
          type(*), dimension(*) :: Rmass, Rmom, Rener
       end function msod_full_diffusion_ijk_mixed
+
+      integer(c_int) function msod_full_diffusion_ijk_closure(porder, nelem, npoin, connec, Ngp, dNgp, He, &
+            gpvol, dlxigp_ip, xgp, invAtoIJK, AtoI, AtoJ, AtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, &
+            mlp_params, used_model, Rmass, Rmom, Rener) bind(C, name='msod_full_diffusion_ijk_closure')
+         import :: c_int, c_double
+         integer(c_int), value :: porder, nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp
+         type(*), dimension(*), intent(in) :: invAtoIJK, AtoI, AtoJ, AtoK
+         real(c_double), value :: Cp, Pr
+         type(*), dimension(*), intent(in) :: rho, u, Tem, mu_fluid, mu_e, mlp_params
+         integer(c_int), intent(out) :: used_model
+         type(*), dimension(*) :: Rmass, Rmom, Rener
+      end function msod_full_diffusion_ijk_closure
    end interface
 
 contains

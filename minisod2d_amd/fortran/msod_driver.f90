@@ -247,18 +247,20 @@ contains
       end do
    end subroutine reference_fields
 
-   ! results_convec.dat / results_diffu.dat (main.f90:586-599), small meshes only
+   ! results_convec.dat / results_diffu.dat (main.f90:586-599), small meshes
+   ! only; one record per node (explicit format: list-directed output may wrap)
    subroutine write_results()
       integer :: ip
+      character(*), parameter :: fmt = '(i0,5(1x,es24.16e3))'
       if (npoin > 200000) return
       open(unit=10, file='results_convec.dat', status='replace')
       do ip = 1, npoin
-         write(10,*) ip, Rmass(ip), Rmom(ip,1), Rmom(ip,2), Rmom(ip,3), Rener(ip)
+         write(10,fmt) ip, Rmass(ip), Rmom(ip,1), Rmom(ip,2), Rmom(ip,3), Rener(ip)
       end do
       close(10)
       open(unit=11, file='results_diffu.dat', status='replace')
       do ip = 1, npoin
-         write(11,*) ip, Dmass(ip), Dmom(ip,1), Dmom(ip,2), Dmom(ip,3), Dener(ip)
+         write(11,fmt) ip, Dmass(ip), Dmom(ip,1), Dmom(ip,2), Dmom(ip,3), Dener(ip)
       end do
       close(11)
    end subroutine write_results

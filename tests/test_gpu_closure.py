@@ -163,7 +163,7 @@ def test_sensitivity_is_exact_derivative(gpu, orc, p, n, jitter):
 @pytest.mark.parametrize("nsamp", [37, 8000])
 def test_gradient_matches_oracle(gpu, orc, nsamp):
     say(f"[gpu] closure gradient nsamp={nsamp}")
-    prob = m.box_problem(3, 3, jitter=0.05)
+    prob = m.box_problem(6, 3, jitter=0.05)            # 13824 element nodes
     F = gpu_features(prob)[:nsamp].copy()
     P = random_params(2, features=F)
     gnu = np.random.default_rng(3).uniform(0.5, 1.5, nsamp).astype(np.float32)
@@ -199,9 +199,11 @@ def test_adam_matches_oracle(gpu, orc):
         m.update_host(Pg, m1, v1)
     finally:
         m.delete(Pg, m1, v1, grad)
-    np.testing.assert_allclose(Pg, Po, rtol=1e-5, atol=1e-7)
-    np.testing.assert_allclose(m1, m2, rtol=1e-5, atol=1e-9)
-    np.testing.assert_allclose(v1, v2, rtol=1e-5, atol=1e-12)
+    # float32 moment updates on the GPU, double then float in the oracle:
+    # equal to float rounding (a few ulp of the largest entries)
+    np.testing.assert_allclose(Pg, Po, rtol=1e-5, atol=1e-6 * np.abs(Po).max())
+    np.testing.assert_allclose(m1, m2, rtol=1e-5, atol=1e-6 * np.abs(m2).max())
+    np.testing.assert_allclose(v1, v2, rtol=1e-5, atol=1e-6 * np.abs(v2).max())
 
 
 def test_loss_matches_numpy(gpu):
