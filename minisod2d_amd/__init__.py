@@ -23,7 +23,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
-HIP_LIB = os.path.join(LIB_DIR, "libmsod2d_hip.so")
+# MSOD2D_HIP_LIB: another build of the kernel library (same ABI), e.g. for tuning
+HIP_LIB = os.environ.get("MSOD2D_HIP_LIB") or os.path.join(LIB_DIR, "libmsod2d_hip.so")
 HOST_LIB = os.path.join(LIB_DIR, "libmsod_host.so")
 
 # status codes (include/msod2d.h)

@@ -161,10 +161,13 @@ def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float =
               log=None) -> ClosurePair:
     """Training pair on n^3 affine hexahedra (see the module docstring)."""
     t0 = time.time()
+    say = log or (lambda *_: None)
     seeds = pair_seeds(seed)
     fine = box_problem(n, pf, jitter=0.0, rho_jump=rho_jump, seeds=seeds)
+    say(f"  n={n}: order-{pf} problem built ({time.time() - t0:.1f} s)")
     fine.mu_sgs[:] = 0.0
     Rcf, Rdf = gpu_residuals(fine)
+    say(f"  n={n}: order-{pf} residuals ({time.time() - t0:.1f} s)")
     coarse = box_problem(n, pt, jitter=0.0, rho_jump=rho_jump, seeds=seeds)
     nodal = np.column_stack([fine.u, fine.q, fine.rho, fine.pr, fine.E, fine.Tem, fine.mu_fluid])
     ft = prolong_fields(fine, coarse, nodal)
@@ -177,7 +180,9 @@ def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float =
     coarse.mu_fluid[:] = ft[:, 10]
     coarse.mu_e[:] = prolong_element_field(fine, coarse, fine.mu_e)
     coarse.mu_sgs[:] = 0.0
+    say(f"  n={n}: fields moved to order {pt} ({time.time() - t0:.1f} s)")
     rt = restrict_residual(fine, coarse, np.column_stack([Rcf[1] + Rdf[1], Rcf[2] + Rdf[2]]))
+    say(f"  n={n}: residual restricted ({time.time() - t0:.1f} s)")
     del fine, Rcf, Rdf, nodal, ft
     Rct, Rd0 = gpu_residuals(coarse)                    # mu_sgs = 0
     target = _mom_ener(rt[:, 0:3], rt[:, 3]) - _mom_ener(Rct[1], Rct[2])
@@ -185,11 +190,39 @@ def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float =
     gap = ((r0 - target) ** 2).sum(axis=1)
     weights = 1.0 / np.maximum(gap, np.finfo(float).tiny)
     meta = dict(n=n, seed=seed, seeds=list(seeds), pf=pf, pt=pt, rho_jump=rho_jump,
-                nelem=coarse.nelem, npoin=coarse.npoin, build_s=round(time.time() - t0, 2))
+                nelem=coarse.nelem, npoin=coarse.npoin, build_s=round(time.time() - t0, 2),
+                fine_norm=np.linalg.norm(_mom_ener(rt[:, 0:3], rt[:, 3]), axis=1).tolist())
     if log:
         log(f"pair n={n} seed={seed}: {coarse.nelem * coarse.tables.nnode} samples, "
             f"{meta['build_s']} s")
     return ClosurePair(coarse, target, r0, weights, meta)
+
+
+def planted_pair(prob: Problem, teacher: np.ndarray) -> ClosurePair:
+    """A pair whose target is the diffusion residual with mu_sgs given by a
+    known ("teacher") network: a closure that can be learnt exactly, to check
+    the trainer end to end."""
+    t = prob.tables
+    prob.mu_sgs[:] = 0.0
+    _, Rd0 = gpu_residuals(prob)
+    feats = np.zeros((prob.nelem * t.nnode, NIN), np.float32)
+    mu = np.zeros((prob.nelem, t.nnode), order="F")
+    ins = [prob.connec, prob.He, prob.gpvol, prob.rho, prob.u]
+    copyin(*ins)
+    create(feats, mu)
+    try:
+        closure_features(prob.nelem, prob.npoin, prob.connec, prob.He, prob.gpvol, t.dlxigp_ip,
+                         t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.rho, prob.u, feats)
+        closure_predict(prob.nelem, t.invAtoIJK, teacher, feats, mu)
+        update_host(mu)
+    finally:
+        delete(*ins, feats, mu)
+    _, Rdt = gpu_residuals(prob, mu_sgs=mu)
+    target = _mom_ener(Rdt[1], Rdt[2])
+    r0 = _mom_ener(Rd0[1], Rd0[2])
+    gap = ((r0 - target) ** 2).sum(axis=1)
+    return ClosurePair(prob, target, r0, 1.0 / np.maximum(gap, np.finfo(float).tiny),
+                       dict(planted=True, nelem=prob.nelem, npoin=prob.npoin))
 
 
 # ---------------------------------------------------------------------------
@@ -342,6 +375,7 @@ def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: in
     vdevs = []
     try:
         vdevs = [_Device(p) for p in (val_pairs or [])]
+        cstar = None
         if init is not None:
             params = init["params"].copy()
             m, v, step0 = init["adam_m"].copy(), init["adam_v"].copy(), init["step"]
@@ -398,7 +432,7 @@ def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: in
         finally:
             delete(params, m, v, grad, gtmp)
         out = dict(params=params, adam_m=m, adam_v=v, step=step0 + steps, history=hist,
-                   val_history=vhist, last_params=params.copy())
+                   val_history=vhist, last_params=params.copy(), constant_optimum=cstar)
         if best is not None:
             out["params"] = best[1]
             out["best_step"] = best[2]
@@ -432,8 +466,57 @@ def evaluate(pair: ClosurePair, params: np.ndarray | None) -> dict:
                 gain=float(1.0 - per.sum() / 4.0))
 
 
+def report(pair: ClosurePair, params: np.ndarray, repeats: int = 10) -> dict:
+    """Error against the projected fine residual with the closure and with
+    mu_sgs = 0, and the cost of the fused closure kernel against the plain
+    diffusion kernel (mu_sgs read from memory) on the same order-pt problem."""
+    from . import (TIMER_DIFFU, TIMER_DIFFU_MLP, set_async, synchronize, timing_enable,
+                   timing_get, timing_reset)
+    ev = evaluate(pair, params)
+    ev0 = evaluate(pair, None)
+    p = pair.coarse
+    t = p.tables
+    R = p.outputs()
+    ins = [p.connec, p.He, p.gpvol, p.rho, p.u, p.Tem, p.mu_fluid, p.mu_e, p.mu_sgs]
+    copyin(*ins, params)
+    create(*R)
+    try:
+        def run(fused):
+            if fused:
+                full_diffusion_ijk_closure(p.nelem, p.npoin, p.connec, t.Ngp, t.dNgp, p.He, p.gpvol,
+                                           t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK,
+                                           p.Cp, p.Pr, p.rho, p.u, p.Tem, p.mu_fluid, p.mu_e, params,
+                                           *R)
+            else:
+                full_diffusion_ijk(*p.diffu_args(*R))
+        cost = {}
+        for fused, timer in ((False, TIMER_DIFFU), (True, TIMER_DIFFU_MLP)):
+            run(fused)
+            synchronize()
+            timing_reset()
+            timing_enable(True)
+            for _ in range(repeats):
+                run(fused)
+            synchronize()
+            timing_enable(False)
+            ms, nl = timing_get(timer)
+            cost["closure_ms" if fused else "plain_ms"] = ms / max(repeats, 1)
+    finally:
+        delete(*ins, params, *R)
+    # relative L2 error of the order-pt residual against the projected fine
+    # residual, per component (momentum x, y, z, energy)
+    fine = np.asarray(pair.meta.get("fine_norm", [np.nan] * 4))
+    err = lambda per: (np.sqrt(np.asarray(per) / pair.weights) / fine).tolist()
+    return dict(n=pair.meta.get("n"), nelem=p.nelem, npoin=p.npoin,
+                loss_model=ev["loss"], loss_zero=ev0["loss"], gain=ev["gain"],
+                gain_per_component=[1 - x for x in ev["per_component"]],
+                rel_err_vs_projected_fine_model=err(ev["per_component"]),
+                rel_err_vs_projected_fine_zero=err(ev0["per_component"]),
+                **cost, cost_ratio=cost["closure_ms"] / cost["plain_ms"])
+
+
 # ---------------------------------------------------------------------------
-# command line: python -m minisod2d_amd.closure train|evaluate ...
+# command line: python -m minisod2d_amd.closure train|evaluate|report ...
 # ---------------------------------------------------------------------------
 def _main(argv=None) -> int:
     from . import init as dev_init
@@ -444,8 +527,14 @@ def _main(argv=None) -> int:
     tr.add_argument("--seed", type=int, default=0)
     tr.add_argument("--steps", type=int, default=600)
     tr.add_argument("--lr", type=float, default=3e-3)
+    tr.add_argument("--val-sizes", type=int, nargs="*", default=[16, 20])
     tr.add_argument("--out", default=DEFAULT_WEIGHTS)
     tr.add_argument("--resume", action="store_true")
+    rp = sub.add_parser("report", help="gain and cost of the closure on one n^3 pair")
+    rp.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    rp.add_argument("--n", type=int, default=96)
+    rp.add_argument("--seed", type=int, default=50)
+    rp.add_argument("--repeats", type=int, default=10)
     ev = sub.add_parser("evaluate")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ev.add_argument("--sizes", type=int, nargs="+", default=[28])
@@ -455,15 +544,24 @@ def _main(argv=None) -> int:
     dev_init(0)
     if a.cmd == "train":
         pairs = [make_pair(n, seed=a.seed + i, log=log) for i, n in enumerate(a.sizes)]
+        val = [make_pair(n, seed=a.seed + 100 + i, log=log) for i, n in enumerate(a.val_sizes)]
         init = load_weights(a.out) if a.resume else None
-        res = train(pairs, steps=a.steps, lr=a.lr, seed=a.seed, init=init, log=log)
+        res = train(pairs, steps=a.steps, lr=a.lr, seed=a.seed, init=init, log=log,
+                    val_pairs=val or None)
         evals = [evaluate(p, res["params"]) for p in pairs]
+        vevals = [evaluate(p, res["params"]) for p in val]
         meta = dict(sizes=a.sizes, seed=a.seed, steps=res["step"], lr=a.lr,
                     samples=int(sum(p.nsamples for p in pairs)),
                     loss_first=res["history"][0], loss_last=res["history"][-1],
-                    train_gain=[e["gain"] for e in evals])
+                    best_step=res.get("best_step"), val_sizes=a.val_sizes,
+                    train_gain=[e["gain"] for e in evals], val_gain=[e["gain"] for e in vevals],
+                    constant_optimum=res.get("constant_optimum"))
         sha = save_weights(a.out, res["params"], res["adam_m"], res["adam_v"], res["step"], meta)
         print(json.dumps(dict(out=a.out, sha256=sha, **meta)), flush=True)
+    elif a.cmd == "report":
+        w = load_weights(a.weights)
+        pair = make_pair(a.n, seed=a.seed, log=log)
+        print(json.dumps(report(pair, w["params"], a.repeats)), flush=True)
     else:
         w = load_weights(a.weights)
         for i, n in enumerate(a.sizes):

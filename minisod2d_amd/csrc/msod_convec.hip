@@ -40,10 +40,10 @@ struct ConvecNode {
 };
 
 template <int NN, typename TG, typename TF>
-__device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_t e, int l,
+__device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_t e, int l, uint32_t gnode,
                                              ConvecNode<TF> &x)
 {
-    const int64_t g = a.plan.conn[e * NN + l];
+    const int64_t g = gnode;
     const int64_t np = a.npoin;
     x.r = a.rho[g];
     x.E = a.E[g];
@@ -56,135 +56,186 @@ __device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_
     x.li = a.plan.lidx[e * NN + l];
 }
 
+// Launch bound: 3 workgroups per CU (up to 168 VGPRs), room for the loads in
+// flight; 4 would force spills.
 template <int P, typename TG, typename TF>
-__global__ void __launch_bounds__(Shape<P>::NT, 4) k_convec(ConvecArgs<TG, TF> a)
+__global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a)
 {
     using S = Shape<P>;
     using T = double;  // arithmetic
-    constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
+    constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX, NPASS = S::NPASS;
     constexpr int NF = 10;
     __shared__ T sD[Q * Q];
     __shared__ T sF[EPB][NF][NN];
     __shared__ double sR[5][UMAX];
+    __shared__ uint32_t sB[kMaxKpb], sU0[kMaxKpb], sU1[kMaxKpb];
+    constexpr int NT = S::NT, UPT = (UMAX + NT - 1) / NT;
 
     const Plan &pl = a.plan;
-    const int64_t b = pl.batches[xcd_slot(blockIdx.x, gridDim.x)];
-    const uint32_t u0 = pl.uoff[b], U = pl.uoff[b + 1] - u0;
+    const int64_t first = (int64_t)xcd_slot(blockIdx.x, gridDim.x) * pl.kpb;
+    const int nbat = (int)(pl.nb - first < pl.kpb ? pl.nb - first : pl.kpb);
+    const int npass = nbat * NPASS;  // passes of this block, over its batches
     const int t = threadIdx.x;
-    const int le = t / NN;
+    const int le = t / NN;  // >= EPB for the few idle threads
     const int l = t - le * NN;
     const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
+    // Loads are unconditional, so that the compiler can count them exactly
+    // and wait for each one only where it is used: idle threads and the
+    // elements past the end of the mesh load a real element, and the result
+    // is not used.
+    const int lec = le < EPB ? le : EPB - 1;
+    auto elem = [&](int64_t bb, int p) -> int64_t {
+        const int64_t e = bb * S::BATCH + p * EPB + lec;
+        return e < a.nelem ? e : a.nelem - 1;
+    };
+    // element of pass g of this block (g clamped to the last pass)
+    auto elem_of = [&](int g) -> int64_t {
+        const int gc = g < npass ? g : npass - 1;
+        return elem((int64_t)sB[gc / NPASS], gc % NPASS);
+    };
 
-    ConvecNode<TF> cur, nxt;
-    {
-        const int64_t e = b * S::BATCH + le;
-        if (le < EPB && e < a.nelem) convec_fetch<NN>(a, e, l, cur);
+    for (int i = t; i < nbat; i += blockDim.x) {
+        const uint32_t bi = pl.batches[first + i];
+        sB[i] = bi;
+        sU0[i] = pl.uoff[bi];
+        sU1[i] = pl.uoff[bi + 1];
     }
     for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
-    for (uint32_t i = t; i < U; i += blockDim.x)
-#pragma unroll
-        for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
 
-#pragma unroll
-    for (int pass = 0; pass < S::NPASS; ++pass) {
-        const int64_t e = b * S::BATCH + pass * EPB + le;
-        const bool active = (le < EPB) && (e < a.nelem);
-        __syncthreads();  // previous pass done with sF (and sR zeroed)
-        if (active) {
-            const T r = cur.r, Ev = cur.E;
-            const T ux = cur.u[0], uy = cur.u[1], uz = cur.u[2];
-            const T k = ux * ux * (T)0.5 + uy * uy * (T)0.5 + uz * uz * (T)0.5;
-            // p = (gamma-1)(E - rho|u|^2/2) when no pressure array is given
-            const T pv = a.pr ? (T)cur.p : a.gamma_m1 * (Ev - r * (T)0.5 * (ux * ux + uy * uy + uz * uz));
-            const T H = Ev + k;
-            const T rH = r * Ev + r * k;
-            sF[le][0][l] = r;
-            sF[le][1][l] = pv;
-            sF[le][2][l] = H;
-            sF[le][3][l] = rH;
-            sF[le][4][l] = ux;
-            sF[le][5][l] = uy;
-            sF[le][6][l] = uz;
-            sF[le][7][l] = cur.q[0];
-            sF[le][8][l] = cur.q[1];
-            sF[le][9][l] = cur.q[2];
-        }
-        if (pass + 1 < S::NPASS) {  // next pass's inputs: in flight during this pass
-            const int64_t en = e + EPB;
-            if (le < EPB && en < a.nelem) convec_fetch<NN>(a, en, l, nxt);
-        }
-        __syncthreads();
-        double v[5] = {0, 0, 0, 0, 0};
-        const uint32_t li = cur.li;
-        if (active) {
-            // the node's own values are read back from LDS where needed, so
-            // that registers stay free for the next pass's loads
-            const TG *geo = a.geo + e * 9 * NN + l;
-            // divergence-type accumulators (coefficient independent of d)
-            T aQ = 0, aU = 0, aRHU = 0, aHU = 0, aQU[3] = {0, 0, 0}, aUU[3] = {0, 0, 0};
-            // gradient-type terms, contracted per direction with the node's own values
-            T sRho = 0, sH = 0, sRH = 0, gP[3] = {0, 0, 0}, qGu[3] = {0, 0, 0}, uGq[3] = {0, 0, 0};
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const T h0 = geo[(0 + 3 * d) * NN], h1 = geo[(1 + 3 * d) * NN], h2 = geo[(2 + 3 * d) * NN];
-                const T Ua = h0 * sF[le][4][l] + h1 * sF[le][5][l] + h2 * sF[le][6][l];
-                const T Qa = h0 * sF[le][7][l] + h1 * sF[le][8][l] + h2 * sF[le][9][l];
-                const int Id = d == 0 ? I : (d == 1 ? J : K);
-                const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
-                const int base = l - Id * stride;
-#pragma unroll 1
-                for (int n = 0; n < Q; ++n) {
-                    const int s = base + n * stride;
-                    const T Dv = sD[Id * Q + n];
-                    const T rs = sF[le][0][s], ps = sF[le][1][s], Hs = sF[le][2][s], rHs = sF[le][3][s];
-                    const T us0 = sF[le][4][s], us1 = sF[le][5][s], us2 = sF[le][6][s];
-                    const T qs0 = sF[le][7][s], qs1 = sF[le][8][s], qs2 = sF[le][9][s];
-                    // the node's own u, q, He column fold into the weights of this source
-                    const T UaD = Ua * Dv, QaD = Qa * Dv;
-                    sRho += UaD * rs;
-                    sRH += UaD * rHs;
-                    sH += QaD * Hs;
-                    gP[0] += (h0 * Dv) * ps; gP[1] += (h1 * Dv) * ps; gP[2] += (h2 * Dv) * ps;
-                    qGu[0] += QaD * us0; qGu[1] += QaD * us1; qGu[2] += QaD * us2;
-                    uGq[0] += UaD * qs0; uGq[1] += UaD * qs1; uGq[2] += UaD * qs2;
-                    const T DU = Dv * (h0 * us0 + h1 * us1 + h2 * us2);
-                    const T DQ = Dv * (h0 * qs0 + h1 * qs1 + h2 * qs2);
-                    aU += DU;
-                    aQ += DQ;
-                    aRHU += DU * rHs;
-                    aHU += DU * Hs;
-                    aQU[0] += DU * qs0; aQU[1] += DU * qs1; aQU[2] += DU * qs2;
-                    aUU[0] += DU * us0; aUU[1] += DU * us1; aUU[2] += DU * us2;
-                }
-            }
-            const T gv = a.gvol[e * NN + l];
-            const T r = sF[le][0][l], H = sF[le][2][l], rH = sF[le][3][l];
-            const T uu[3] = {sF[le][4][l], sF[le][5][l], sF[le][6][l]};
-            const T qq[3] = {sF[le][7][l], sF[le][8][l], sF[le][9][l]};
-            const T Rm = (T)0.5 * (aQ + r * aU) + (T)0.5 * sRho;
-            const T Re = (T)0.25 * (aRHU + H * aQ + rH * aU + r * aHU) + (T)0.25 * (sH + sRH + H * sRho);
-            T Rmo[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                Rmo[i] = (T)0.25 * (aQU[i] + uu[i] * aQ + qq[i] * aU + r * aUU[i] + uu[i] * sRho +
-                                    qGu[i] + uGq[i]) + gP[i];
-            v[0] = (double)(gv * Rm);
-            v[1] = (double)(gv * Rmo[0]);
-            v[2] = (double)(gv * Rmo[1]);
-            v[3] = (double)(gv * Rmo[2]);
-            v[4] = (double)(gv * Re);
-        }
-        add_pass<UMAX, EPB>(sR, le, active, active ? li : 0u, v);
-        if (pass + 1 < S::NPASS) cur = nxt;
+    // prologue: nodal values of pass 0, connectivity of pass 1
+    ConvecNode<TF> nd[2];
+    uint32_t gnext;  // global node of the next pass's element node
+    {
+        const int64_t b0 = pl.batches[first];
+        const int64_t e0 = elem(b0, 0);
+        convec_fetch<NN>(a, e0, l, pl.conn[e0 * NN + l], nd[0]);
+        const int64_t b1 = npass > 1 ? (int64_t)pl.batches[first + 1 / NPASS] : b0;
+        const int64_t e1 = npass > 1 ? elem(b1, 1 % NPASS) : e0;
+        gnext = pl.conn[e1 * NN + l];
     }
-    write_batch<UMAX>(pl, u0, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
+
+    for (int kb = 0; kb < nbat; ++kb) {
+        __syncthreads();  // sB, sD visible; the previous batch's write phase is done with sR
+        const int64_t b = sB[kb];
+        const uint32_t u0 = sU0[kb], U = sU1[kb] - u0;
+        uint32_t wn[UPT];  // this thread's entries of the batch's node list, for the write phase
+#pragma unroll
+        for (int j = 0; j < UPT; ++j) {
+            const uint32_t i = t + j * NT;
+            wn[j] = pl.unode[u0 + (i < U ? i : U - 1)];
+        }
+        for (uint32_t i = t; i < U; i += blockDim.x)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
+#pragma unroll
+        for (int p = 0; p < NPASS; ++p) {
+            ConvecNode<TF> &cur = nd[p & 1];
+            ConvecNode<TF> &nxt = nd[(p + 1) & 1];
+            const int g = kb * NPASS + p;
+            const int64_t e = b * S::BATCH + p * EPB + le;
+            const bool active = (le < EPB) && (e < a.nelem);
+            if (p > 0) __syncthreads();  // the previous pass is done with sF
+            if (active) {
+                const T r = cur.r, Ev = cur.E;
+                const T ux = cur.u[0], uy = cur.u[1], uz = cur.u[2];
+                const T k = ux * ux * (T)0.5 + uy * uy * (T)0.5 + uz * uz * (T)0.5;
+                // p = (gamma-1)(E - rho|u|^2/2) when no pressure array is given
+                const T pv = a.pr ? (T)cur.p
+                                  : a.gamma_m1 * (Ev - r * (T)0.5 * (ux * ux + uy * uy + uz * uz));
+                const T H = Ev + k;
+                const T rH = r * Ev + r * k;
+                sF[le][0][l] = r;
+                sF[le][1][l] = pv;
+                sF[le][2][l] = H;
+                sF[le][3][l] = rH;
+                sF[le][4][l] = ux;
+                sF[le][5][l] = uy;
+                sF[le][6][l] = uz;
+                sF[le][7][l] = cur.q[0];
+                sF[le][8][l] = cur.q[1];
+                sF[le][9][l] = cur.q[2];
+            }
+            const uint32_t li = cur.li;
+            // issued now, used later: He and gpvol of this pass (after the
+            // barrier), the next pass's nodal values (next pass), the
+            // connectivity of the pass after it
+            const int64_t ec = elem(b, p);
+            const TG *geo = a.geo + ec * 9 * NN + l;
+            T hh[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) hh[c] = geo[c * NN];
+            const T gv = a.gvol[ec * NN + l];
+            convec_fetch<NN>(a, elem_of(g + 1), l, gnext, nxt);
+            gnext = pl.conn[elem_of(g + 2) * NN + l];
+            __syncthreads();
+            double v[5] = {0, 0, 0, 0, 0};
+            if (active) {
+                // divergence-type accumulators (coefficient independent of d)
+                T aQ = 0, aU = 0, aRHU = 0, aHU = 0, aQU[3] = {0, 0, 0}, aUU[3] = {0, 0, 0};
+                // gradient-type terms, contracted per direction with the node's own values
+                T sRho = 0, sH = 0, sRH = 0, gP[3] = {0, 0, 0}, qGu[3] = {0, 0, 0}, uGq[3] = {0, 0, 0};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const T h0 = hh[0 + 3 * d], h1 = hh[1 + 3 * d], h2 = hh[2 + 3 * d];
+                    const T Ua = h0 * sF[le][4][l] + h1 * sF[le][5][l] + h2 * sF[le][6][l];
+                    const T Qa = h0 * sF[le][7][l] + h1 * sF[le][8][l] + h2 * sF[le][9][l];
+                    const int Id = d == 0 ? I : (d == 1 ? J : K);
+                    const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
+                    const int base = l - Id * stride;
+#pragma unroll 1
+                    for (int n = 0; n < Q; ++n) {
+                        const int s = base + n * stride;
+                        const T Dv = sD[Id * Q + n];
+                        const T rs = sF[le][0][s], ps = sF[le][1][s], Hs = sF[le][2][s], rHs = sF[le][3][s];
+                        const T us0 = sF[le][4][s], us1 = sF[le][5][s], us2 = sF[le][6][s];
+                        const T qs0 = sF[le][7][s], qs1 = sF[le][8][s], qs2 = sF[le][9][s];
+                        // the node's own u, q, He column fold into the weights of this source
+                        const T UaD = Ua * Dv, QaD = Qa * Dv;
+                        sRho += UaD * rs;
+                        sRH += UaD * rHs;
+                        sH += QaD * Hs;
+                        gP[0] += (h0 * Dv) * ps; gP[1] += (h1 * Dv) * ps; gP[2] += (h2 * Dv) * ps;
+                        qGu[0] += QaD * us0; qGu[1] += QaD * us1; qGu[2] += QaD * us2;
+                        uGq[0] += UaD * qs0; uGq[1] += UaD * qs1; uGq[2] += UaD * qs2;
+                        const T DU = Dv * (h0 * us0 + h1 * us1 + h2 * us2);
+                        const T DQ = Dv * (h0 * qs0 + h1 * qs1 + h2 * qs2);
+                        aU += DU;
+                        aQ += DQ;
+                        aRHU += DU * rHs;
+                        aHU += DU * Hs;
+                        aQU[0] += DU * qs0; aQU[1] += DU * qs1; aQU[2] += DU * qs2;
+                        aUU[0] += DU * us0; aUU[1] += DU * us1; aUU[2] += DU * us2;
+                    }
+                }
+                const T r = sF[le][0][l], H = sF[le][2][l], rH = sF[le][3][l];
+                const T uu[3] = {sF[le][4][l], sF[le][5][l], sF[le][6][l]};
+                const T qq[3] = {sF[le][7][l], sF[le][8][l], sF[le][9][l]};
+                const T Rm = (T)0.5 * (aQ + r * aU) + (T)0.5 * sRho;
+                const T Re = (T)0.25 * (aRHU + H * aQ + rH * aU + r * aHU) + (T)0.25 * (sH + sRH + H * sRho);
+                T Rmo[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    Rmo[i] = (T)0.25 * (aQU[i] + uu[i] * aQ + qq[i] * aU + r * aUU[i] + uu[i] * sRho +
+                                        qGu[i] + uGq[i]) + gP[i];
+                v[0] = (double)(gv * Rm);
+                v[1] = (double)(gv * Rmo[0]);
+                v[2] = (double)(gv * Rmo[1]);
+                v[3] = (double)(gv * Rmo[2]);
+                v[4] = (double)(gv * Re);
+            }
+            add_pass<UMAX, EPB>(sR, pl.mode, le, active, li, v);
+        }
+        if (pl.mode == kAtomic) __syncthreads();  // every LDS atomic of the batch done
+        write_batch_nodes<UMAX, NT>(pl.mode, wn, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
+        if (NPASS & 1) nd[0] = nd[1];  // odd pass count: the next batch starts from slot 0
+    }
 }
 
 template <int P, typename TG, typename TF>
 static hipError_t launch_p(const ConvecArgs<TG, TF> &a, hipStream_t s)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_convec<P, TG, TF>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s, a);
+    hipLaunchKernelGGL((k_convec<P, TG, TF>), dim3(plan_blocks(a.plan)), dim3(Shape<P>::NT), 0, s, a);
     return hipGetLastError();
 }
 

@@ -80,9 +80,16 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
     auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf);  // contravariant fluxes of one direction
 
     const Plan &pl = a.plan;
+    constexpr int NT = S::NT, UPT = (UMAX + NT - 1) / NT;
     const int64_t b = pl.batches[xcd_slot(blockIdx.x, gridDim.x)];
     const uint32_t u0 = pl.uoff[b], U = pl.uoff[b + 1] - u0;
     const int t = threadIdx.x;
+    uint32_t wn[UPT];  // this thread's entries of the batch's node list, for the write phase
+#pragma unroll
+    for (int j = 0; j < UPT; ++j) {
+        const uint32_t i = t + j * NT;
+        wn[j] = pl.unode[u0 + (i < U ? i : U - 1)];
+    }
     const int le = t / NN;
     const int l = t - le * NN;
     const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
@@ -229,10 +236,11 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
             v[3] = (double)W[5];
             v[4] = (double)(W[1] + kappa * W[2]);
         }
-        add_pass<UMAX, EPB>(sR, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
+        add_pass<UMAX, EPB>(sR, pl.mode, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
         if (pass + 1 < S::NPASS) cur = nxt;
     }
-    write_batch<UMAX>(pl, u0, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
+    __syncthreads();  // every node sum of the batch done
+    write_batch_nodes<UMAX, NT>(pl.mode, wn, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 
 template <int P, typename TG, typename TF, bool CLOSURE = false>

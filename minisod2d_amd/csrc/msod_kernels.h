@@ -19,7 +19,9 @@
 //   uoff  [nbatch+1]    node list of batch b: unode[uoff[b] .. uoff[b+1])
 //   unode [..]          sorted distinct global nodes of each batch; bit 31 set
 //                       when this batch is the first writer of the node
-//   batches[nb]         the batches processed by one launch
+//   batches[nb]         the batches processed by one launch; a workgroup takes
+//                       kpb consecutive ones, one after the other, with the
+//                       loads of the next pass always in flight
 // Within a batch the element-node values are summed in a fixed order
 // (add_pass).  Two ways to combine the batches' partial sums at nodes shared
 // between batches (mode):
@@ -36,6 +38,8 @@
 namespace msod {
 
 constexpr int kMaxP = 8;
+constexpr int kMaxKpb = 64;           // batches per workgroup, at most
+constexpr int kBatchesPerBlock = 8;   // batches per workgroup (measured best of 1..16 at p=4)
 enum AssemblyMode { kColored = 0, kAtomic = 1 };
 
 template <int P>
@@ -68,7 +72,11 @@ struct Plan {
     const uint32_t *batches;
     int64_t nb;
     int mode;
+    int kpb;  // consecutive batches of the list per workgroup
 };
+
+// Workgroups of a launch: each takes kpb consecutive batches of the list.
+inline unsigned plan_blocks(const Plan &p) { return (unsigned)((p.nb + p.kpb - 1) / p.kpb); }
 
 // Kernel arguments.  Storage types: TG for the element-sized arrays (geo,
 // gvol; in the diffusion also mu_e, mu_sgs), TF for the nodal fields.  The
@@ -133,17 +141,22 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n)
 }
 
 // Write phase shared by the kernels: the batch's node sums sR[c][i]
-// (c = Rmass, Rmom x3, Rener) go to the outputs.
-template <int UMAX>
-__device__ __forceinline__ void write_batch(const Plan &pl, uint32_t u0, uint32_t U,
-                                            const double (*sR)[UMAX], double *Rmass,
-                                            double *Rmom, double *Rener, int64_t npoin)
+// (c = Rmass, Rmom x3, Rener) go to the outputs.  Thread t holds entries t,
+// t + NT, ... of the batch's node list in registers (wn), loaded when the
+// batch started.
+template <int UMAX, int NT>
+__device__ __forceinline__ void write_batch_nodes(int mode, const uint32_t (&wn)[(UMAX + NT - 1) / NT],
+                                                  uint32_t U, const double (*sR)[UMAX], double *Rmass,
+                                                  double *Rmom, double *Rener, int64_t npoin)
 {
-    for (uint32_t i = threadIdx.x; i < U; i += blockDim.x) {
-        const uint32_t w = pl.unode[u0 + i];
+#pragma unroll
+    for (int j = 0; j < (UMAX + NT - 1) / NT; ++j) {
+        const uint32_t i = threadIdx.x + j * NT;
+        if (i >= U) break;
+        const uint32_t w = wn[j];
         const int64_t g = w & 0x7fffffffu;
         double *dst[5] = {Rmass + g, Rmom + g, Rmom + npoin + g, Rmom + 2 * npoin + g, Rener + g};
-        if (pl.mode == kAtomic) {
+        if (mode == kAtomic) {
 #pragma unroll
             for (int c = 0; c < 5; ++c) unsafeAtomicAdd(dst[c], sR[c][i]);
         } else if (w >> 31) {
@@ -161,10 +174,21 @@ __device__ __forceinline__ void write_batch(const Plan &pl, uint32_t u0, uint32_
 // one element are distinct, so no two threads of a round touch the same sum,
 // and the summation order (pass, then element, within the batch) is fixed.
 // Every thread of the block calls it (uniform control flow, barriers).
+// In the atomic mode, whose global sums have no fixed order anyway, the
+// batch sums use LDS atomics instead (no rounds); the caller synchronises
+// before the batch is written.
 template <int UMAX, int EPB>
-__device__ __forceinline__ void add_pass(double (*sR)[UMAX], int le, bool active, uint32_t li,
+__device__ __forceinline__ void add_pass(double (*sR)[UMAX], int mode, int le, bool active, uint32_t li,
                                          const double (&v)[5])
 {
+    if (mode == kAtomic) {
+        if (active) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                __hip_atomic_fetch_add(&sR[c][li], v[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        return;
+    }
 #pragma unroll 1
     for (int k = 0; k < EPB; ++k) {
         if (active && le == k) {

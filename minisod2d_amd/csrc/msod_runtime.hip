@@ -26,17 +26,22 @@ namespace msod {
 // ---------------------------------------------------------------------------
 // layout kernels
 // ---------------------------------------------------------------------------
-__global__ void k_zero_outputs(double *a, double *b, double *c, int64_t n)
+// Zero n doubles with 16-byte stores (p is 16-byte aligned: hipMalloc bases
+// and whole arrays are passed).
+__global__ void k_zero(double *p, int64_t n)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 5 * n; i += stride) {
-        if (i < n)
-            a[i] = 0.0;
-        else if (i < 4 * n)
-            b[i - n] = 0.0;
-        else
-            c[i - 4 * n] = 0.0;
-    }
+    const int64_t n2 = n / 2;
+    double2 *q = reinterpret_cast<double2 *>(p);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride)
+        q[i] = make_double2(0.0, 0.0);
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) p[n - 1] = 0.0;
+}
+
+__global__ void k_zero_unaligned(double *p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
 }
 
 __global__ void k_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
@@ -78,9 +83,20 @@ static unsigned grid_for(int64_t n, int bs)
 
 hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64_t n, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_zero_outputs, dim3(grid_for(5 * n, 256)), dim3(256), 0, s, Rmass, Rmom,
-                       Rener, n);
-    return hipGetLastError();
+    const int64_t len[3] = {n, 3 * n, n};
+    double *ptr[3] = {Rmass, Rmom, Rener};
+    for (int k = 0; k < 3; ++k) {
+        const bool aligned = ((uintptr_t)ptr[k] & 15u) == 0;
+        // 16-byte stores need a 16-byte aligned base; otherwise one double at a time
+        if (aligned) {
+            hipLaunchKernelGGL(k_zero, dim3(grid_for((len[k] + 1) / 2, 256)), dim3(256), 0, s, ptr[k], len[k]);
+        } else {
+            hipLaunchKernelGGL(k_zero_unaligned, dim3(grid_for(len[k], 256)), dim3(256), 0, s, ptr[k], len[k]);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
@@ -564,6 +580,7 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     pl.uoff = P.uoff;
     pl.unode = P.unode;
     pl.mode = g.assembly;
+    pl.kpb = kBatchesPerBlock;
     const int64_t NP = a.npoin;
     {
         TimedLaunch tz(0);

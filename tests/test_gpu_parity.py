@@ -346,7 +346,7 @@ def test_fortran_driver(gpu, tmp_path, args, case):
     """The Fortran timer driver (drop-in elem_convec/elem_diffu modules over the
     C ABI) reproduces the reference's residuals and writes timers.dat."""
     import subprocess
-    exe = os.path.join(os.path.dirname(m.HIP_LIB), "miniSOD")
+    exe = os.path.join(m.LIB_DIR, "miniSOD")
     say(f"[gpu] Fortran driver {' '.join(args)}")
     r = subprocess.run([exe, *args, "--nruns", "3", "--quiet"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
