@@ -29,6 +29,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import minisod2d_amd as m  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X, bytes/s
+CALIB_BYTES = 2 << 30  # device-to-device copy that scales the HBM byte counters
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_convec_diffu_128cubed_p4.md")
+TRAFFIC_TAG = "traffic-json: "
+KERNEL_RE = {"full_convec_ijk": r"k_convec<(\d+), double, double>",
+             "full_diffusion_ijk": r"k_diffu<(\d+), double, double, false>",
+             "zero_fill": r"msod::k_zero\("}
 
 
 def log(*a):
@@ -43,6 +49,95 @@ def canonical_bytes(E, N, wg, wf, we=None):
     bc = E * (4 + 10 * wg) + N * (9 * wf + 40)
     bd = E * (4 + 10 * wg + 2 * we) + N * (6 * wf + 40)
     return bc, bd
+
+
+def calibrate_copy():
+    """One device-to-device copy of CALIB_BYTES through the HIP runtime the
+    kernel library uses (a blit kernel: CALIB_BYTES read, CALIB_BYTES
+    written), so that a counter run sees a dispatch of known HBM traffic."""
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(CALIB_BYTES)) != 0:
+            raise RuntimeError("hipMalloc of the calibration buffers failed")
+    try:
+        if hip.hipMemset(src, 1, ctypes.c_size_t(CALIB_BYTES)) != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(CALIB_BYTES), 3) != 0:  # device to device
+            raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
+def summarize_pmc(fetch_csv, write_csv, n, porder):
+    """HBM bytes per dispatch of the residual kernels from one FETCH_SIZE and
+    one WRITE_SIZE counter run (rocprofv3 --pmc ... --output-format csv),
+    each scaled by what the counter read for the calibration copy (the
+    largest device-to-device blit, CALIB_BYTES each way)."""
+    import csv
+    import re
+
+    def per_dispatch(path, counter):
+        rows = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            key = (r["Dispatch_Id"], r["Kernel_Name"])
+            rows[key] = rows.get(key, 0.0) + float(r["Counter_Value"]) * 1024.0  # KiB
+        return rows
+
+    out = {"calibration": {}, "kernels": {}}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = per_dispatch(path, counter)
+        copies = [v for (d, k), v in rows.items() if "copyBuffer" in k]
+        if not copies:
+            raise SystemExit(f"{path}: no device-to-device copy dispatch (run with --calibrate-copy)")
+        factor = max(copies) / CALIB_BYTES
+        if not 0.5 < factor < 2.0:
+            raise SystemExit(f"{path}: largest copy reads {max(copies):.4g} bytes of {counter}, "
+                             f"not the {CALIB_BYTES}-byte calibration copy")
+        out["calibration"][counter] = {"copy_bytes": CALIB_BYTES, "counter_bytes": max(copies),
+                                       "factor": factor}
+        for name, pat in KERNEL_RE.items():
+            vals = [v for (d, k), v in rows.items() if re.search(pat, k)]
+            if not vals:
+                continue
+            ent = out["kernels"].setdefault(name, {})
+            ent["dispatches"] = len(vals)
+            ent[counter.lower() + "_raw_bytes"] = sum(vals) / len(vals)
+            ent[counter.lower() + "_bytes"] = sum(vals) / len(vals) / factor
+    prob_n = (n * porder + 1) ** 3
+    E = n ** 3 * (porder + 1) ** 3
+    bc, bd = canonical_bytes(E, prob_n, 8, 8)
+    algo = {"full_convec_ijk": bc, "full_diffusion_ijk": bd}
+    for name, ent in out["kernels"].items():
+        if "fetch_size_bytes" in ent and "write_size_bytes" in ent:
+            ent["traffic_bytes"] = ent["fetch_size_bytes"] + ent["write_size_bytes"]
+            if name in algo:
+                ent["algorithmic_bytes"] = algo[name]
+                ent["traffic_over_algorithmic"] = ent["traffic_bytes"] / algo[name]
+    out["workload"] = {"n": n, "porder": porder, "npoin": prob_n, "elem_nodes": E}
+    return out
+
+
+def load_traffic(n, porder, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    the 128^3 porder-4 workload, or None for any other workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith(TRAFFIC_TAG):
+                    rec = json.loads(line[len(TRAFFIC_TAG):])
+                    w = rec["workload"]
+                    if (w["n"], w["porder"]) != (n, porder):
+                        return None
+                    return rec["kernels"].get(kernel, {}).get("traffic_bytes")
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
 
 
 TIMERS = {"fp64": (m.TIMER_CONVEC, m.TIMER_DIFFU),
@@ -174,7 +269,14 @@ def main():
                     help="node assembly of the reported run")
     ap.add_argument("--compare-assembly", action="store_true",
                     help="also time the other assembly mode (FP64)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 2 GiB device-to-device copy first (HBM counter calibration)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print HBM bytes per dispatch from two counter runs and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(*args.summarize_pmc, args.n, args.porder)))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported here "
               f"(the element-partitioned multi-GPU form is a design, not a build)", file=sys.stderr)
@@ -188,6 +290,8 @@ def main():
     log(f"generated npoin={N} elem-nodes={E} in {time.time()-t0:.1f}s")
 
     m.init(0)
+    if args.calibrate_copy:
+        calibrate_copy()
     R, D = prob.outputs(), prob.outputs()
     ins = prob.device_inputs()
     t0 = time.time()
@@ -241,7 +345,7 @@ def main():
                    "npoin": N, "nelem": prob.nelem, "porder": p},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None},
+                     "traffic": load_traffic(n, p, dom)},
         "kernels_ms": {"convec": r64["convec_ms"], "diffu": r64["diffu_ms"],
                        "zero_fill": r64["zero_ms"]},
         "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,

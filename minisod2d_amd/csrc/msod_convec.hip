@@ -53,7 +53,7 @@ __device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_
         x.q[d] = a.q[g + d * np];
     }
     x.p = a.pr ? a.pr[g] : (TF)0;
-    x.li = a.plan.lidx[e * NN + l];
+    x.li = __builtin_nontemporal_load(&a.plan.lidx[e * NN + l]);
 }
 
 // Launch bound: 3 workgroups per CU (up to 168 VGPRs), room for the loads in
@@ -108,10 +108,10 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
     {
         const int64_t b0 = pl.batches[first];
         const int64_t e0 = elem(b0, 0);
-        convec_fetch<NN>(a, e0, l, pl.conn[e0 * NN + l], nd[0]);
+        convec_fetch<NN>(a, e0, l, __builtin_nontemporal_load(&pl.conn[e0 * NN + l]), nd[0]);
         const int64_t b1 = npass > 1 ? (int64_t)pl.batches[first + 1 / NPASS] : b0;
         const int64_t e1 = npass > 1 ? elem(b1, 1 % NPASS) : e0;
-        gnext = pl.conn[e1 * NN + l];
+        gnext = __builtin_nontemporal_load(&pl.conn[e1 * NN + l]);
     }
 
     for (int kb = 0; kb < nbat; ++kb) {
@@ -158,15 +158,17 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
             const uint32_t li = cur.li;
             // issued now, used later: He and gpvol of this pass (after the
             // barrier), the next pass's nodal values (next pass), the
-            // connectivity of the pass after it
+            // connectivity of the pass after it.  The element arrays are read
+            // once: non-temporal loads keep them from evicting the gathered
+            // nodal lines, which neighbouring elements read again, from L2.
             const int64_t ec = elem(b, p);
             const TG *geo = a.geo + ec * 9 * NN + l;
             T hh[9];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) hh[c] = geo[c * NN];
-            const T gv = a.gvol[ec * NN + l];
+            for (int c = 0; c < 9; ++c) hh[c] = __builtin_nontemporal_load(&geo[c * NN]);
+            const T gv = __builtin_nontemporal_load(&a.gvol[ec * NN + l]);
             convec_fetch<NN>(a, elem_of(g + 1), l, gnext, nxt);
-            gnext = pl.conn[elem_of(g + 2) * NN + l];
+            gnext = __builtin_nontemporal_load(&pl.conn[elem_of(g + 2) * NN + l]);
             __syncthreads();
             double v[5] = {0, 0, 0, 0, 0};
             if (active) {

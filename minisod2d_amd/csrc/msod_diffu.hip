@@ -48,7 +48,7 @@ template <int NN, bool CLOSURE, typename TG, typename TF>
 __device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t e, int l,
                                             DiffuNode<TG, TF> &x)
 {
-    const int64_t g = a.plan.conn[e * NN + l];
+    const int64_t g = __builtin_nontemporal_load(&a.plan.conn[e * NN + l]);
     const int64_t np = a.npoin;
     x.r = a.rho[g];
     x.T_ = a.Tem[g];
@@ -135,11 +135,11 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
         T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         T tauU[3] = {0, 0, 0}, gv = 0, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (active) {
-            const TG *geo = a.geo + e * 9 * NN + l;
+            const TG *geo = a.geo + e * 9 * NN + l;  // read once: non-temporal, spares L2
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) h[i][d] = geo[(i + 3 * d) * NN];
+                for (int i = 0; i < 3; ++i) h[i][d] = __builtin_nontemporal_load(&geo[(i + 3 * d) * NN]);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int Id = d == 0 ? I : (d == 1 ? J : K);
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
                 tauU[i] = acc - lam * uu[i];
                 tau[i][i] -= lam;
             }
-            gv = a.gvol[e * NN + l];
+            gv = __builtin_nontemporal_load(&a.gvol[e * NN + l]);
         }
 
         // ---- pass 2: transposed sweeps, one reference direction at a time -
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
             v[3] = (double)W[5];
             v[4] = (double)(W[1] + kappa * W[2]);
         }
-        add_pass<UMAX, EPB>(sR, pl.mode, le, active, active ? pl.lidx[e * NN + l] : 0u, v);
+        add_pass<UMAX, EPB>(sR, pl.mode, le, active, active ? (uint32_t)__builtin_nontemporal_load(&pl.lidx[e * NN + l]) : 0u, v);
         if (pass + 1 < S::NPASS) cur = nxt;
     }
     __syncthreads();  // every node sum of the batch done

@@ -143,7 +143,11 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n)
 // Write phase shared by the kernels: the batch's node sums sR[c][i]
 // (c = Rmass, Rmom x3, Rener) go to the outputs.  Thread t holds entries t,
 // t + NT, ... of the batch's node list in registers (wn), loaded when the
-// batch started.
+// batch started.  Atomic mode: every entry is added atomically onto the
+// zero-filled outputs (agent-scope FP64 atomics, performed past the L2; a
+// plain store of the nodes no other batch holds was tried and is much
+// slower, since those lines also take atomics).  Coloured mode: the first
+// writer stores, the others add.
 template <int UMAX, int NT>
 __device__ __forceinline__ void write_batch_nodes(int mode, const uint32_t (&wn)[(UMAX + NT - 1) / NT],
                                                   uint32_t U, const double (*sR)[UMAX], double *Rmass,
