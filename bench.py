@@ -345,7 +345,8 @@ def main():
                    "npoin": N, "nelem": prob.nelem, "porder": p},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": load_traffic(n, p, dom)},
+                     "traffic": load_traffic(n, p, dom),
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)},
         "kernels_ms": {"convec": r64["convec_ms"], "diffu": r64["diffu_ms"],
                        "zero_fill": r64["zero_ms"]},
         "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,
