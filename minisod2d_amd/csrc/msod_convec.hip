@@ -8,21 +8,23 @@
 //     writes 10 nodal values to LDS:
 //       rho, p, H = E + k, rhoH = rho E + rho k, u(3), q(3)     (k = |u|^2/2)
 //  2. sweeps: for each reference direction d the thread runs the 1-D
-//     derivative D[I_d][n] along its grid line (q LDS reads per value).
-//     Rather than differentiating the 39 flux components the reference forms,
-//     the He column of the TARGET node is folded into the fluxes at the
-//     source nodes:  U_d(s) = sum_i He(i,d)u_i(s), Q_d(s) = sum_i He(i,d)q_i(s).
-//     Since He is taken at the target node,
-//       div(q (x) u)_i = sum_d D_d[ q_i U_d ],  div u = sum_d D_d[U_d], ...
-//     which needs 20 accumulators per direction and keeps only 10 values per
-//     node in LDS; the node's own u, q, H then enter the non-divergence
-//     terms through U_d(a), Q_d(a) (sum_i q_i dE/dx_i = sum_d Q_d(a) D_d[E]).
+//     derivative D[I_d][n] along its grid line.  Rather than differentiating
+//     the 39 flux components the reference forms, the He column of the
+//     TARGET node is folded into the fluxes at the source nodes:
+//     U_d(s) = sum_i He(i,d)u_i(s).  Since He is taken at the target node,
+//       div(q (x) u)_i = sum_d D_d[ q_i U_d ],  div u = sum_d He(:,d).D_d[u], ...
+//     Per line the thread forms the plain 1-D derivatives of p, u, q and the
+//     derivatives of the products with U_d; the node's own r, u, q, H and He
+//     column then contract them (eleven accumulators over the three
+//     directions; the sweep code is shared by the directions).  LDS holds the
+//     ten values of a node contiguously, so a source costs one address and
+//     five 16-byte reads.
 //     The result equals the reference's algebra exactly in real arithmetic;
 //     rounding differs at the 1e-16 level.
 //  3. residual x gpvol, added into the batch's per-node sums in LDS in a
 //     fixed order (add_pass).
 // After the passes the batch's distinct nodes are written once each
-// (write_batch: plain store / read-add-store per colour, or FP64 atomics).
+// (write_batch_nodes: plain store / read-add-store per colour, or FP64 atomics).
 //
 // Storage types TG (geometry) and TF (nodal fields), FP64 arithmetic
 // throughout (msod_kernels.h).  Without a pressure array the pressure is
@@ -66,7 +68,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX, NPASS = S::NPASS;
     constexpr int NF = 10;
     __shared__ T sD[Q * Q];
-    __shared__ T sF[EPB][NF][NN];
+    __shared__ __attribute__((aligned(16))) T sF[EPB][NN][NF];  // node-major: one base address per source
     __shared__ double sR[5][UMAX];
     __shared__ uint32_t sB[kMaxKpb], sU0[kMaxKpb], sU1[kMaxKpb];
     constexpr int NT = S::NT, UPT = (UMAX + NT - 1) / NT;
@@ -144,16 +146,16 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
                                   : a.gamma_m1 * (Ev - r * (T)0.5 * (ux * ux + uy * uy + uz * uz));
                 const T H = Ev + k;
                 const T rH = r * Ev + r * k;
-                sF[le][0][l] = r;
-                sF[le][1][l] = pv;
-                sF[le][2][l] = H;
-                sF[le][3][l] = rH;
-                sF[le][4][l] = ux;
-                sF[le][5][l] = uy;
-                sF[le][6][l] = uz;
-                sF[le][7][l] = cur.q[0];
-                sF[le][8][l] = cur.q[1];
-                sF[le][9][l] = cur.q[2];
+                sF[le][l][0] = r;
+                sF[le][l][1] = pv;
+                sF[le][l][2] = H;
+                sF[le][l][3] = rH;
+                sF[le][l][4] = ux;
+                sF[le][l][5] = uy;
+                sF[le][l][6] = uz;
+                sF[le][l][7] = cur.q[0];
+                sF[le][l][8] = cur.q[1];
+                sF[le][l][9] = cur.q[2];
             }
             const uint32_t li = cur.li;
             // issued now, used later: He and gpvol of this pass (after the
@@ -172,53 +174,64 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
             __syncthreads();
             double v[5] = {0, 0, 0, 0, 0};
             if (active) {
-                // divergence-type accumulators (coefficient independent of d)
-                T aQ = 0, aU = 0, aRHU = 0, aHU = 0, aQU[3] = {0, 0, 0}, aUU[3] = {0, 0, 0};
-                // gradient-type terms, contracted per direction with the node's own values
-                T sRho = 0, sH = 0, sRH = 0, gP[3] = {0, 0, 0}, qGu[3] = {0, 0, 0}, uGq[3] = {0, 0, 0};
-#pragma unroll
+                // Accumulators, each a sum over the three directions and the
+                // q sources of one line (r, H, rH, u, q: the node's own values):
+                //   aU = div u, aQ = div q                       (from the line derivatives)
+                //   aE = sum D[(U.)(rH + r H)],  aM_i = sum D[(U.)(q_i + r u_i)]
+                //   sRho = sum U_d D_d[r],  sE = sum Q_d D_d[H] + U_d D_d[rH]
+                //   mM_i = sum Q_d D_d[u_i] + U_d D_d[q_i] + 4 He(i,d) D_d[p]
+                const T r = (T)cur.r;
+                T aQ = 0, aU = 0, aE = 0, aM[3] = {0, 0, 0};
+                T sRho = 0, sE = 0, mM[3] = {0, 0, 0};
+#pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
-                    const T h0 = hh[0 + 3 * d], h1 = hh[1 + 3 * d], h2 = hh[2 + 3 * d];
-                    const T Ua = h0 * sF[le][4][l] + h1 * sF[le][5][l] + h2 * sF[le][6][l];
-                    const T Qa = h0 * sF[le][7][l] + h1 * sF[le][8][l] + h2 * sF[le][9][l];
+                    // one copy of the sweep code for the three directions
+                    const T h0 = d == 0 ? hh[0] : (d == 1 ? hh[3] : hh[6]);
+                    const T h1 = d == 0 ? hh[1] : (d == 1 ? hh[4] : hh[7]);
+                    const T h2 = d == 0 ? hh[2] : (d == 1 ? hh[5] : hh[8]);
                     const int Id = d == 0 ? I : (d == 1 ? J : K);
                     const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
                     const int base = l - Id * stride;
+                    const T Ua = h0 * sF[le][l][4] + h1 * sF[le][l][5] + h2 * sF[le][l][6];
+                    const T Qa = h0 * sF[le][l][7] + h1 * sF[le][l][8] + h2 * sF[le][l][9];
+                    // 1-D derivatives along this line of the stored values
+                    T dp = 0, du[3] = {0, 0, 0}, dq[3] = {0, 0, 0};
 #pragma unroll 1
                     for (int n = 0; n < Q; ++n) {
                         const int s = base + n * stride;
                         const T Dv = sD[Id * Q + n];
-                        const T rs = sF[le][0][s], ps = sF[le][1][s], Hs = sF[le][2][s], rHs = sF[le][3][s];
-                        const T us0 = sF[le][4][s], us1 = sF[le][5][s], us2 = sF[le][6][s];
-                        const T qs0 = sF[le][7][s], qs1 = sF[le][8][s], qs2 = sF[le][9][s];
-                        // the node's own u, q, He column fold into the weights of this source
+                        const T rs = sF[le][s][0], ps = sF[le][s][1], Hs = sF[le][s][2], rHs = sF[le][s][3];
+                        const T us0 = sF[le][s][4], us1 = sF[le][s][5], us2 = sF[le][s][6];
+                        const T qs0 = sF[le][s][7], qs1 = sF[le][s][8], qs2 = sF[le][s][9];
                         const T UaD = Ua * Dv, QaD = Qa * Dv;
                         sRho += UaD * rs;
-                        sRH += UaD * rHs;
-                        sH += QaD * Hs;
-                        gP[0] += (h0 * Dv) * ps; gP[1] += (h1 * Dv) * ps; gP[2] += (h2 * Dv) * ps;
-                        qGu[0] += QaD * us0; qGu[1] += QaD * us1; qGu[2] += QaD * us2;
-                        uGq[0] += UaD * qs0; uGq[1] += UaD * qs1; uGq[2] += UaD * qs2;
+                        sE += QaD * Hs + UaD * rHs;
+                        dp += Dv * ps;
+                        du[0] += Dv * us0; du[1] += Dv * us1; du[2] += Dv * us2;
+                        dq[0] += Dv * qs0; dq[1] += Dv * qs1; dq[2] += Dv * qs2;
+                        // products with the target's contravariant velocity
                         const T DU = Dv * (h0 * us0 + h1 * us1 + h2 * us2);
-                        const T DQ = Dv * (h0 * qs0 + h1 * qs1 + h2 * qs2);
-                        aU += DU;
-                        aQ += DQ;
-                        aRHU += DU * rHs;
-                        aHU += DU * Hs;
-                        aQU[0] += DU * qs0; aQU[1] += DU * qs1; aQU[2] += DU * qs2;
-                        aUU[0] += DU * us0; aUU[1] += DU * us1; aUU[2] += DU * us2;
+                        aE += DU * (rHs + r * Hs);
+                        aM[0] += DU * (qs0 + r * us0);
+                        aM[1] += DU * (qs1 + r * us1);
+                        aM[2] += DU * (qs2 + r * us2);
                     }
+                    const T dp4 = (T)4 * dp;
+                    mM[0] += Qa * du[0] + Ua * dq[0] + h0 * dp4;
+                    mM[1] += Qa * du[1] + Ua * dq[1] + h1 * dp4;
+                    mM[2] += Qa * du[2] + Ua * dq[2] + h2 * dp4;
+                    aU += h0 * du[0] + h1 * du[1] + h2 * du[2];
+                    aQ += h0 * dq[0] + h1 * dq[1] + h2 * dq[2];
                 }
-                const T r = sF[le][0][l], H = sF[le][2][l], rH = sF[le][3][l];
-                const T uu[3] = {sF[le][4][l], sF[le][5][l], sF[le][6][l]};
-                const T qq[3] = {sF[le][7][l], sF[le][8][l], sF[le][9][l]};
+                const T H = sF[le][l][2], rH = sF[le][l][3];
+                const T uu[3] = {sF[le][l][4], sF[le][l][5], sF[le][l][6]};
+                const T qq[3] = {sF[le][l][7], sF[le][l][8], sF[le][l][9]};
                 const T Rm = (T)0.5 * (aQ + r * aU) + (T)0.5 * sRho;
-                const T Re = (T)0.25 * (aRHU + H * aQ + rH * aU + r * aHU) + (T)0.25 * (sH + sRH + H * sRho);
+                const T Re = (T)0.25 * (aE + H * aQ + rH * aU) + (T)0.25 * (sE + H * sRho);
                 T Rmo[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
-                    Rmo[i] = (T)0.25 * (aQU[i] + uu[i] * aQ + qq[i] * aU + r * aUU[i] + uu[i] * sRho +
-                                        qGu[i] + uGq[i]) + gP[i];
+                    Rmo[i] = (T)0.25 * (aM[i] + uu[i] * aQ + qq[i] * aU + uu[i] * sRho + mM[i]);
                 v[0] = (double)(gv * Rm);
                 v[1] = (double)(gv * Rmo[0]);
                 v[2] = (double)(gv * Rmo[1]);

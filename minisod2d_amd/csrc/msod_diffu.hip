@@ -20,7 +20,7 @@
 //    reference multiplies kappa_e inside the sum; same value, other rounding).
 // Each element node's five values are added into the batch's per-node sums
 // in LDS in a fixed order (add_pass); the batch's distinct nodes are then
-// written once (write_batch).
+// written once (write_batch_nodes).
 //
 // Storage types TG (geometry, mu_e, mu_sgs) and TF (nodal fields), FP64
 // arithmetic throughout (msod_kernels.h).  CLOSURE (FP64 storage only):

@@ -39,7 +39,7 @@ namespace msod {
 
 constexpr int kMaxP = 8;
 constexpr int kMaxKpb = 64;           // batches per workgroup, at most
-constexpr int kBatchesPerBlock = 8;   // batches per workgroup (measured best of 1..16 at p=4)
+constexpr int kBatchesPerBlock = 8;   // batches per convection workgroup (measured best of 1..16 at p=4)
 enum AssemblyMode { kColored = 0, kAtomic = 1 };
 
 template <int P>
