@@ -41,7 +41,6 @@ namespace msod {
 template <typename TG, typename TF>
 struct DiffuNode {
     TF r, T_, muf, u[3];
-    TG mue, mus;
 };
 
 template <int NN, bool CLOSURE, typename TG, typename TF>
@@ -55,9 +54,25 @@ __device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t 
     x.muf = a.mu_fluid[g];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x.u[d] = a.u[g + d * np];
-    const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
-    x.mue = a.mu_e[ge];
-    x.mus = (!CLOSURE && a.mu_sgs) ? a.mu_sgs[ge] : (TG)0;  // NULL: closure fallback
+}
+
+// mu_e and mu_sgs of a batch's elements into LDS.  Their reference layout
+// (nelem,ngaus) puts the values of one element node 8*nelem bytes apart, so
+// a wave reading one element's nodes would touch 64 lines (and pages) per
+// load; here four neighbouring lanes read four consecutive elements of one
+// node, and the batches that follow in mesh order reuse the lines.
+template <int NN, int BATCH, int NT, bool CLOSURE, typename TG, typename TF>
+__device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64_t b, TG (*sMu)[BATCH][NN])
+{
+    for (int i = threadIdx.x; i < BATCH * NN; i += NT) {
+        const int eb = i % BATCH, l = i / BATCH;
+        const int64_t e = b * BATCH + eb;
+        if (e < a.nelem) {
+            const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
+            sMu[0][eb][l] = a.mu_e[ge];
+            sMu[1][eb][l] = (!CLOSURE && a.mu_sgs) ? a.mu_sgs[ge] : (TG)0;  // NULL: closure fallback
+        }
+    }
 }
 
 template <int P, typename TG, typename TF, bool CLOSURE>
@@ -75,6 +90,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
     __shared__ T sD[Q * Q];
     __shared__ __attribute__((aligned(16))) unsigned char sbuf[kBuf];
     __shared__ double sR[5][UMAX];
+    __shared__ TG sMu[2][S::BATCH][NN];  // mu_e, mu_sgs of the batch
     __shared__ typename std::conditional<CLOSURE, MlpShared, char>::type smlp;
     auto sA = reinterpret_cast<T (*)[5][NN]>(sbuf);  // rho, T, u0, u1, u2
     auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf);  // contravariant fluxes of one direction
@@ -100,6 +116,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
         const int64_t e = b * S::BATCH + le;
         if (le < EPB && e < a.nelem) diffu_fetch<NN, CLOSURE>(a, e, l, cur);
     }
+    diffu_stage_mu<NN, S::BATCH, NT, CLOSURE>(a, b, sMu);
     for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
     for (uint32_t i = t; i < U; i += blockDim.x)
 #pragma unroll
@@ -111,12 +128,12 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
         const int64_t e = b * S::BATCH + pass * EPB + le;
         const bool active = (le < EPB) && (e < a.nelem);
         T r = 0, muf = 0, mue = 0, mus = 0, uu[3] = {0, 0, 0};
-        __syncthreads();  // previous pass done with sA/sG (and sR zeroed)
+        __syncthreads();  // previous pass done with sA/sG (and sR zeroed, sMu staged)
         if (active) {
             r = cur.r;
             muf = cur.muf;
-            mue = cur.mue;
-            mus = cur.mus;
+            mue = (T)sMu[0][pass * EPB + le][l];
+            mus = (T)sMu[1][pass * EPB + le][l];
             sA[le][0][l] = r;
             sA[le][1][l] = cur.T_;
             for (int d = 0; d < 3; ++d) {
