@@ -96,7 +96,7 @@ def summarize_pmc(fetch_csv, write_csv, n, porder):
         if not copies:
             raise SystemExit(f"{path}: no device-to-device copy dispatch (run with --calibrate-copy)")
         factor = max(copies) / CALIB_BYTES
-        if not 0.5 < factor < 2.0:
+        if not 0.4 <= factor <= 2.5:
             raise SystemExit(f"{path}: largest copy reads {max(copies):.4g} bytes of {counter}, "
                              f"not the {CALIB_BYTES}-byte calibration copy")
         out["calibration"][counter] = {"copy_bytes": CALIB_BYTES, "counter_bytes": max(copies),

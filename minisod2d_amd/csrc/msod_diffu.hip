@@ -56,17 +56,18 @@ __device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t 
     for (int d = 0; d < 3; ++d) x.u[d] = a.u[g + d * np];
 }
 
-// mu_e and mu_sgs of a batch's elements into LDS.  Their reference layout
-// (nelem,ngaus) puts the values of one element node 8*nelem bytes apart, so
-// a wave reading one element's nodes would touch 64 lines (and pages) per
-// load; here four neighbouring lanes read four consecutive elements of one
-// node, and the batches that follow in mesh order reuse the lines.
-template <int NN, int BATCH, int NT, bool CLOSURE, typename TG, typename TF>
-__device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64_t b, TG (*sMu)[BATCH][NN])
+// mu_e and mu_sgs of ME consecutive elements from e0 on into LDS.  Their
+// reference layout (nelem,ngaus) puts the values of one element node
+// 8*nelem bytes apart, so a wave reading one element's nodes would touch 64
+// lines (and pages) per load; here ME neighbouring lanes read ME
+// consecutive elements of one node, and the batches that follow in mesh
+// order reuse the lines.
+template <int NN, int ME, int NT, bool CLOSURE, typename TG, typename TF>
+__device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64_t e0, TG (*sMu)[ME][NN])
 {
-    for (int i = threadIdx.x; i < BATCH * NN; i += NT) {
-        const int eb = i % BATCH, l = i / BATCH;
-        const int64_t e = b * BATCH + eb;
+    for (int i = threadIdx.x; i < ME * NN; i += NT) {
+        const int eb = i % ME, l = i / ME;
+        const int64_t e = e0 + eb;
         if (e < a.nelem) {
             const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
             sMu[0][eb][l] = a.mu_e[ge];
@@ -90,7 +91,15 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
     __shared__ T sD[Q * Q];
     __shared__ __attribute__((aligned(16))) unsigned char sbuf[kBuf];
     __shared__ double sR[5][UMAX];
-    __shared__ TG sMu[2][S::BATCH][NN];  // mu_e, mu_sgs of the batch
+    // mu_e, mu_sgs of the batch, staged once; of the pass, staged per pass,
+    // where the batch's would cost a workgroup per CU (four fit 160 KB of LDS)
+    constexpr size_t kLdsRest = sizeof(T) * Q * Q + kBuf + sizeof(double) * 5 * UMAX +
+                                (CLOSURE ? sizeof(MlpShared) : 0);
+    constexpr size_t kLdsCu = 160 * 1024 / 4;
+    constexpr bool kMuPerPass = kLdsRest + 2 * sizeof(TG) * S::BATCH * NN > kLdsCu &&
+                                kLdsRest + 2 * sizeof(TG) * EPB * NN <= kLdsCu;
+    constexpr int ME = kMuPerPass ? EPB : S::BATCH;
+    __shared__ TG sMu[2][ME][NN];
     __shared__ typename std::conditional<CLOSURE, MlpShared, char>::type smlp;
     auto sA = reinterpret_cast<T (*)[5][NN]>(sbuf);  // rho, T, u0, u1, u2
     auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf);  // contravariant fluxes of one direction
@@ -116,7 +125,7 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
         const int64_t e = b * S::BATCH + le;
         if (le < EPB && e < a.nelem) diffu_fetch<NN, CLOSURE>(a, e, l, cur);
     }
-    diffu_stage_mu<NN, S::BATCH, NT, CLOSURE>(a, b, sMu);
+    if (!kMuPerPass) diffu_stage_mu<NN, ME, NT, CLOSURE>(a, b * S::BATCH, sMu);
     for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
     for (uint32_t i = t; i < U; i += blockDim.x)
 #pragma unroll
@@ -128,12 +137,14 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_diffu(DiffuArgs<TG, TF> a, con
         const int64_t e = b * S::BATCH + pass * EPB + le;
         const bool active = (le < EPB) && (e < a.nelem);
         T r = 0, muf = 0, mue = 0, mus = 0, uu[3] = {0, 0, 0};
+        // the previous pass read its sMu entries before its pass-2 barriers
+        if (kMuPerPass) diffu_stage_mu<NN, ME, NT, CLOSURE>(a, b * S::BATCH + pass * EPB, sMu);
         __syncthreads();  // previous pass done with sA/sG (and sR zeroed, sMu staged)
         if (active) {
             r = cur.r;
             muf = cur.muf;
-            mue = (T)sMu[0][pass * EPB + le][l];
-            mus = (T)sMu[1][pass * EPB + le][l];
+            mue = (T)sMu[0][kMuPerPass ? le : pass * EPB + le][l];
+            mus = (T)sMu[1][kMuPerPass ? le : pass * EPB + le][l];
             sA[le][0][l] = r;
             sA[le][1][l] = cur.T_;
             for (int d = 0; d < 3; ++d) {

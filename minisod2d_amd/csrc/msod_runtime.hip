@@ -26,8 +26,10 @@ namespace msod {
 // ---------------------------------------------------------------------------
 // layout kernels
 // ---------------------------------------------------------------------------
-// Zero n doubles with 16-byte stores (p is 16-byte aligned: hipMalloc bases
-// and whole arrays are passed).
+// Zero n doubles with 16-byte stores, one per thread (p is 16-byte aligned:
+// hipMalloc bases and whole arrays are passed).  A grid covering the array
+// in one trip measured faster than grid-stride loops of 1024 or 4096
+// workgroups.
 __global__ void k_zero(double *p, int64_t n)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -89,7 +91,8 @@ hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64
         const bool aligned = ((uintptr_t)ptr[k] & 15u) == 0;
         // 16-byte stores need a 16-byte aligned base; otherwise one double at a time
         if (aligned) {
-            hipLaunchKernelGGL(k_zero, dim3(grid_for((len[k] + 1) / 2, 256)), dim3(256), 0, s, ptr[k], len[k]);
+            const int64_t blocks = std::min<int64_t>(len[k] / 2 / 256 + 1, 0x7fffffff);
+            hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, ptr[k], len[k]);
         } else {
             hipLaunchKernelGGL(k_zero_unaligned, dim3(grid_for(len[k], 256)), dim3(256), 0, s, ptr[k], len[k]);
         }
