@@ -191,6 +191,42 @@ def timed(prob, R, D, steps, warmup, variant, f32=None):
                 zero_ms=zc[0] / max(zc[1], 1), launches_per_call=c[1] / max(steps, 1))
 
 
+def time_geometry(prob, threads):
+    """elem_jacobian on the GPU (msod_elem_jacobian, outputs in device-only
+    mirrors) against the host Fortran routine on `threads` cores."""
+    t = prob.tables
+    nn = t.nnode
+    He = np.empty(prob.He.shape, order="F")
+    gp = np.empty(prob.gpvol.shape, order="F")
+    m.copyin(prob.coord, t.dNgp, t.wgp)
+    m.create(He, gp)
+    try:
+        m.timing_reset()
+        m.timing_enable(True)
+        for _ in range(2):  # the first call builds nothing lasting: same work each time
+            nbad = m.elem_jacobian(t.porder, prob.nelem, prob.npoin, prob.connec, prob.coord,
+                                   t.dNgp, t.wgp, gp, He)
+        m.timing_enable(False)
+        ms, n = m.timing_get(m.TIMER_GEOMETRY)
+    finally:
+        m.delete(prob.coord, t.dNgp, t.wgp, He, gp)
+    H = m.host_lib()
+    H.msodh_set_threads(threads)
+    He2 = np.empty(prob.He.shape, order="F")
+    gp2 = np.empty(prob.gpvol.shape, order="F")
+    t0 = time.perf_counter()
+    H.msodh_elem_jacobian(prob.nelem, prob.npoin, nn, prob.connec.ctypes.data, prob.coord.ctypes.data,
+                          t.dNgp.ctypes.data, t.wgp.ctypes.data, gp2.ctypes.data, He2.ctypes.data)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    del He2, gp2
+    E = prob.nelem * nn
+    nbytes = E * (4 + 80) + prob.npoin * 24
+    dev_ms = ms / max(n, 1)
+    return {"device_ms": dev_ms, "host_ms": host_ms, "host_threads": threads, "nbad": nbad,
+            "bytes": nbytes, "device_GBps": nbytes / (dev_ms * 1e-3) / 1e9,
+            "note": "elem_jacobian: He, gpvol from coord on the GPU vs the host Fortran routine"}
+
+
 def sampled_parity(prob, out, nsample=3000, seed=7):
     """Residuals at randomly chosen nodes against the oracle run on exactly
     the elements that touch those nodes (each sampled node then receives all
@@ -264,6 +300,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-geometry", action="store_true", help="skip the elem_jacobian timing")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--assembly", choices=("colored", "atomic"), default="atomic",
                     help="node assembly of the reported run")
@@ -313,6 +350,12 @@ def main():
         parity = sampled_parity(prob, R + D)
         log(f"sampled parity {parity}")
 
+    geometry = None
+    if not args.no_geometry:
+        log("timing the device geometry (elem_jacobian)")
+        geometry = time_geometry(prob, min(args.threads, os.cpu_count() or 1))
+        log(f"geometry {geometry}")
+
     split = {}
     if not args.no_mixed:
         f32 = prob.fp32()
@@ -351,6 +394,7 @@ def main():
                        "zero_fill": r64["zero_ms"]},
         "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,
         "parity_sampled_rel_l2": parity,
+        "geometry": geometry,
         "assembly": args.assembly,
     }
     if other is not None:

@@ -90,9 +90,26 @@ int msod_timing_enable(int enable);
 /* kernel: 0 zero-fill, 1 convec, 2 diffu, 3 convec mixed, 4 diffu mixed,
  *         5 diffu learned closure, 6 closure training kernels (features,
  *         predict, sensitivity, gradient, adam, loss), 7 convec geo32,
- *         8 diffu geo32.  Returns accumulated ms and launch count. */
+ *         8 diffu geo32, 9 geometry (msod_elem_jacobian).  Returns
+ *         accumulated ms and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
+
+/* ---- geometry ------------------------------------------------------------
+ * elem_jacobian (src/fem/sources/jacobian_oper.f90:7-136), which the
+ * reference driver runs on the device once per mesh (main.f90:292-295):
+ * He and gpvol of every element node from the node coordinates,
+ *   Je(i,j) = sum_a dNgp(i,a,g) coord(connec(e,a),j),
+ *   He = cofactor(Je)^T / det(Je),  gpvol = wgp(g) det(Je),
+ * with the reference's operation order for det and the cofactors.
+ * coord(npoin,3), dNgp(3,nnode,ngaus), wgp(ngaus); He(3,3,ngaus,nelem) and
+ * gpvol(1,ngaus,nelem) are written in place (host-present or device).
+ * Synchronous.  *nbad (if not NULL) receives the number of element nodes
+ * with det(Je) <= 0, which the reference does not check; connec entries
+ * outside 1..npoin return MSOD_ERR_MESH. */
+int msod_elem_jacobian(int porder, int nelem, int npoin, const int32_t *connec, const double *coord,
+                       const double *dNgp, const double *wgp, double *gpvol, double *He,
+                       int64_t *nbad);
 
 /* ---- residual kernels --------------------------------------------------- */
 

@@ -40,7 +40,7 @@ MSOD_ERR_MODEL = 8
 
 # timer ids of msod_timing_get
 (TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP,
- TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32) = range(9)
+ TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32, TIMER_GEOMETRY) = range(10)
 
 # the C-ABI entry points declared in include/msod2d.h (checked by the tests)
 ABI_SYMBOLS = (
@@ -54,7 +54,7 @@ ABI_SYMBOLS = (
     "msod_full_convec_ijk_geo32", "msod_full_diffusion_ijk_geo32",
     "msod_full_diffusion_ijk_closure", "msod_closure_features", "msod_closure_predict",
     "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
-    "msod_closure_loss",
+    "msod_closure_loss", "msod_elem_jacobian",
 )
 
 
@@ -111,6 +111,7 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_closure_gradient.argtypes = [ctypes.c_int64] + [_vp] * 4
         L.msod_closure_adam.argtypes = [_vp] * 4 + [_i, _d, _d, _d, _d]
         L.msod_closure_loss.argtypes = [_i] + [_vp] * 6
+        L.msod_elem_jacobian.argtypes = [_i, _i, _i] + [_vp] * 7
         _hip = L
     return _hip
 
@@ -227,6 +228,20 @@ def timing_get(kernel: int) -> tuple[float, int]:
 
 def set_diffusion_coefficients(c_rho: float = 1.0, c_ener: float = 1.0) -> None:
     _check(hip_lib().msod_set_diffusion_coefficients(c_rho, c_ener))
+
+
+# ---------------------------------------------------------------------------
+# geometry, reference argument list
+# ---------------------------------------------------------------------------
+def elem_jacobian(porder, nelem, npoin, connec, coord, dNgp, wgp, gpvol, He) -> int:
+    """elem_jacobian (jacobian_oper.f90:7-136) on the GPU: fills He(3,3,ngaus,nelem)
+    and gpvol(1,ngaus,nelem) from coord(npoin,3); returns the number of element
+    nodes with det(Je) <= 0.  Arrays: host-present (copyin/create) or device."""
+    nbad = ctypes.c_int64(0)
+    _check(hip_lib().msod_elem_jacobian(int(porder), int(nelem), int(npoin),
+                                        *map(_ptr, (connec, coord, dNgp, wgp, gpvol, He)),
+                                        ctypes.byref(nbad)))
+    return int(nbad.value)
 
 
 # ---------------------------------------------------------------------------

@@ -4,6 +4,7 @@
 // layouts derived once per mesh, table validation and the kernel calls.
 #include "msod2d.h"
 #include "msod_closure.h"
+#include "msod_geom.h"
 #include "msod_kernels.h"
 #include "msod_mlp.h"
 #include "msod_plan.h"
@@ -197,7 +198,7 @@ struct DerivedGeo {
     void *geo, *gvol;
 };
 
-constexpr int kNumTimers = 9;
+constexpr int kNumTimers = 10;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -1095,6 +1096,111 @@ int msod_set_assembly(int mode)
         return fail(MSOD_ERR_ARG, "assembly mode %d", mode);
     g.assembly = mode == MSOD_ASSEMBLY_ATOMIC ? kAtomic : kColored;
     return MSOD_OK;
+}
+
+extern "C++" {
+// Device copy of a small host vector, freed when the holder goes out of scope.
+struct DevTemp {
+    void *p = nullptr;
+    ~DevTemp()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+template <class V>
+int dev_temp(const std::vector<V> &h, DevTemp &d, const char *what)
+{
+    HIPCHK(hipMalloc(&d.p, sizeof(V) * std::max<size_t>(1, h.size())), what);
+    if (!h.empty()) HIPCHK(hipMemcpy(d.p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice), what);
+    return MSOD_OK;
+}
+}  // extern "C++"
+
+int msod_elem_jacobian(int porder, int nelem, int npoin, const int32_t *connec, const double *coord,
+                       const double *dNgp, const double *wgp, double *gpvol, double *He, int64_t *nbad)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    int st = check_sizes(porder, nelem, npoin);
+    if (st) return st;
+    const int64_t nn = (int64_t)(porder + 1) * (porder + 1) * (porder + 1), NE = nelem, NP = npoin;
+    const void *dc, *dx, *dv, *dh;
+    uint64_t vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vx))) return st;
+    if ((st = resolve(coord, sizeof(double) * NP * 3, "coord", &dx, &vx))) return st;
+    if ((st = resolve(gpvol, sizeof(double) * NE * nn, "gpvol", &dv, &vx))) return st;
+    if ((st = resolve(He, sizeof(double) * NE * nn * 9, "He", &dh, &vx))) return st;
+    // nonzero pattern of dNgp(3,nnode,ngaus), read on the host
+    std::vector<char> t0, t1;
+    const char *pd, *pw;
+    if ((st = host_view(dNgp, sizeof(double) * 3 * nn * nn, t0, &pd, "dNgp"))) return st;
+    if ((st = host_view(wgp, sizeof(double) * nn, t1, &pw, "wgp"))) return st;
+    const double *D = (const double *)pd;
+    std::vector<uint16_t> nzn(nn * 3, 0);
+    int mz = 1;
+    for (int64_t gg = 0; gg < nn; ++gg)
+        for (int d = 0; d < 3; ++d) {
+            int c = 0;
+            for (int64_t a = 0; a < nn; ++a) c += D[d + 3 * (a + nn * gg)] != 0.0;
+            nzn[gg * 3 + d] = (uint16_t)c;
+            mz = std::max(mz, c);
+        }
+    std::vector<uint16_t> nza(nn * 3 * mz, 0);
+    std::vector<double> nzw(nn * 3 * mz, 0.0);
+    for (int64_t gg = 0; gg < nn; ++gg)
+        for (int d = 0; d < 3; ++d) {
+            int k = 0;
+            for (int64_t a = 0; a < nn; ++a) {
+                const double w = D[d + 3 * (a + nn * gg)];
+                if (w != 0.0) {
+                    nza[(gg * 3 + d) * mz + k] = (uint16_t)a;
+                    nzw[(gg * 3 + d) * mz + k] = w;
+                    ++k;
+                }
+            }
+        }
+    std::vector<double> w(pw ? (const double *)pw : nullptr, pw ? (const double *)pw + nn : nullptr);
+    std::vector<unsigned long long> counts(2, 0ull);
+    DevTemp da, dw, dn, dg, dk;
+    if ((st = dev_temp(nza, da, "geometry tables"))) return st;
+    if ((st = dev_temp(nzw, dw, "geometry tables"))) return st;
+    if ((st = dev_temp(nzn, dn, "geometry tables"))) return st;
+    if ((st = dev_temp(w, dg, "geometry tables"))) return st;
+    if ((st = dev_temp(counts, dk, "geometry counters"))) return st;
+    GeomArgs a;
+    a.connec = (const int32_t *)dc;
+    a.coord = (const double *)dx;
+    a.wgp = (const double *)dg.p;
+    a.nza = (const uint16_t *)da.p;
+    a.nzw = (const double *)dw.p;
+    a.nzn = (const uint16_t *)dn.p;
+    a.nelem = NE;
+    a.npoin = NP;
+    a.mz = mz;
+    a.He = (double *)dh;
+    a.gpvol = (double *)dv;
+    a.counts = (unsigned long long *)dk.p;
+    TimedLaunch tk(9);
+    HIPCHK(launch_elem_jacobian(porder, a, g.stream), "geometry kernel launch");
+    tk.done();
+    // synchronous: the temporaries are freed on return and the counts read
+    HIPCHK(hipMemcpyAsync(counts.data(), dk.p, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost,
+                          g.stream),
+           "geometry counters");
+    HIPCHK(hipStreamSynchronize(g.stream), "geometry kernel");
+    // layouts derived from He and gpvol are stale now
+    for (const void *ptr : {(const void *)He, (const void *)gpvol}) {
+        uintptr_t base;
+        if (Region *r = find_region(ptr, &base))
+            r->version = g.next_version++;
+        else
+            drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+    }
+    if (counts[0])
+        return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%d)",
+                    counts[0], npoin);
+    if (nbad) *nbad = (int64_t)counts[1];
+    return finish_call("geometry kernel");
 }
 
 int msod_set_diffusion_coefficients(double c_rho, double c_ener)

@@ -1,6 +1,7 @@
 ! Drop-in replacements of the reference's kernel modules:
-!   module elem_convec  (kernels/sources/elem_convec.f90:198-211)
-!   module elem_diffu   (kernels/sources/elem_diffu.f90:10-23)
+!   module elem_convec    (kernels/sources/elem_convec.f90:198-211)
+!   module elem_diffu     (kernels/sources/elem_diffu.f90:10-23)
+!   module jacobian_oper  (fem/sources/jacobian_oper.f90:7-19)
 ! Same module and procedure names and the same argument lists, so a caller
 ! written for the reference compiles unchanged; the bodies run the gfx950
 ! kernels through the C ABI.  Differences a caller can see:
@@ -139,3 +140,46 @@ contains
    end subroutine full_diffusion_ijk_closure
 
 end module elem_diffu
+
+module jacobian_oper
+
+   use iso_c_binding
+   use msod_device, only: msod_jacobian_device, msod_check
+   implicit none
+   private
+   public :: elem_jacobian
+
+contains
+
+   ! He and gpvol of every element node on the GPU.  Like the reference,
+   ! which runs it on its `enter data` copies (main.f90:280-295), every
+   ! array must be present on the device (msod_copyin of connec, coord,
+   ! dNgp, wgp; msod_copyin or msod_create of gpvol and He); the results
+   ! stay there until msod_update_host.  porder is read from the shape of
+   ! dNgp(3,nnode,ngaus).  A non-positive det(Je), which the reference does
+   ! not check, stops with a message.
+   subroutine elem_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: coord(:,:), dNgp(:,:,:), wgp(:)
+      real(8), intent(out), contiguous :: gpvol(:,:,:), He(:,:,:,:)
+      integer :: nn, porder
+      integer(c_int64_t) :: nbad
+      nn = size(dNgp, 2)
+      porder = nint(real(nn, 8)**(1d0/3d0)) - 1
+      if ((porder+1)**3 /= nn .or. size(connec,1) /= nelem .or. size(connec,2) /= nn .or. &
+          size(coord,1) < npoin .or. size(He,3) /= nn .or. size(He,4) < nelem .or. &
+          size(gpvol,2) /= nn .or. size(gpvol,3) < nelem .or. size(wgp) < nn) then
+         write(*,'(a,i0,a,i0)') 'msod: elem_jacobian: array shapes do not match nelem=', nelem, &
+              ' npoin=', npoin
+         error stop 2
+      end if
+      call msod_check(msod_jacobian_device(int(porder, c_int), nelem, npoin, connec, coord, dNgp, wgp, &
+                      gpvol, He, nbad), 'elem_jacobian')
+      if (nbad > 0) then
+         write(*,'(a,i0,a)') 'msod: elem_jacobian: ', nbad, ' element nodes with det(Je) <= 0'
+         error stop 3
+      end if
+   end subroutine elem_jacobian
+
+end module jacobian_oper

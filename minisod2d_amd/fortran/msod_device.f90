@@ -19,7 +19,7 @@ module msod_device
    public :: msod_delete, msod_sync, msod_check, msod_set_async_mode, msod_set_assembly_mode
    public :: msod_full_convec_ijk, msod_full_diffusion_ijk, msod_full_convec_ijk_mixed
    public :: msod_full_diffusion_ijk_mixed, msod_set_diffusion_coefficients
-   public :: msod_full_diffusion_ijk_closure
+   public :: msod_full_diffusion_ijk_closure, msod_jacobian_device
 
    interface
       integer(c_int) function msod_init(device) bind(C, name='msod_init')
@@ -129,6 +129,15 @@ module msod_device
          integer(c_int), intent(out) :: used_model
          type(*), dimension(*) :: Rmass, Rmom, Rener
       end function msod_full_diffusion_ijk_closure
+
+      integer(c_int) function msod_jacobian_device(porder, nelem, npoin, connec, coord, dNgp, wgp, &
+            gpvol, He, nbad) bind(C, name='msod_elem_jacobian')
+         import :: c_int, c_int64_t
+         integer(c_int), value :: porder, nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, coord, dNgp, wgp
+         type(*), dimension(*) :: gpvol, He
+         integer(c_int64_t), intent(out) :: nbad
+      end function msod_jacobian_device
    end interface
 
 contains

@@ -8,10 +8,14 @@
 !
 ! Usage: miniSOD [--n N] [--nelem NE] [--porder P] [--nruns R] [--jitter J]
 !                [--rho-jump F] [--mixed] [--assembly colored|atomic] [--quiet]
+!                [--gpu-geometry]
 !   --n N      conforming N x N x N box of the unit cube (default 8)
 !   --nelem NE the reference's own case instead: NE disjoint elements with
 !              its initial conditions (main.f90:257-413)
 !   --mixed    time the mixed-precision kernels (FP32 state and geometry)
+!   --gpu-geometry  compute He and gpvol on the device (elem_jacobian of the
+!              drop-in module jacobian_oper) on the device copies, as the
+!              reference does (main.f90:280-295); FP64 kernels only
 program miniSOD
 
    use iso_c_binding
@@ -22,11 +26,12 @@ program miniSOD
    use msod_device
    use elem_convec, only: full_convec_ijk
    use elem_diffu, only: full_diffusion_ijk
+   use jacobian_oper, only: elem_jacobian
    implicit none
 
    integer :: n = 8, nelem_ref = 0, porder = 4, nruns = 10, nn, q, i, nbad, assembly = 0
    real(8) :: jitter = 0d0, rho_jump = 0d0
-   logical :: mixed = .false., quiet = .false.
+   logical :: mixed = .false., quiet = .false., gpu_geometry = .false.
    integer(8) :: nelem8, npoin8
    integer :: nelem, npoin
    integer(c_int), allocatable :: connec(:,:), invAtoIJK(:,:,:), AtoIJK(:), AtoI(:), AtoJ(:), AtoK(:)
@@ -123,6 +128,13 @@ program miniSOD
    else
       call msod_copyin(He, c_sizeof(He))
       call msod_copyin(gpvol, c_sizeof(gpvol))
+      if (gpu_geometry) then
+         ! the device copies of He and gpvol are recomputed from the coordinates
+         call msod_copyin(coord, c_sizeof(coord))
+         call msod_copyin(dNgp, c_sizeof(dNgp))
+         call msod_copyin(wgp, c_sizeof(wgp))
+         call elem_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He)
+      end if
       call msod_copyin(u, c_sizeof(u))
       call msod_copyin(qm, c_sizeof(qm))
       call msod_copyin(rho, c_sizeof(rho))
@@ -277,6 +289,8 @@ contains
             mixed = .true.
          case ('--quiet')
             quiet = .true.
+         case ('--gpu-geometry')
+            gpu_geometry = .true.
          case ('--n', '--nelem', '--porder', '--nruns', '--jitter', '--rho-jump', '--assembly')
             if (k == nargs) stop 'missing value'
             call get_command_argument(k+1, v)
@@ -302,6 +316,7 @@ contains
          k = k + 1
       end do
       if (porder < 1 .or. porder > 8) stop 'porder must be 1..8'
+      if (gpu_geometry .and. mixed) stop '--gpu-geometry computes FP64 geometry: not with --mixed'
    end subroutine parse_args
 
 end program miniSOD

@@ -341,6 +341,7 @@ def test_shuffled_elements_and_unreferenced_nodes(gpu):
 @pytest.mark.parametrize("args,case", [
     (["--nelem", "6", "--porder", "4"], "case_refdriver_p4"),
     (["--n", "8", "--porder", "3"], "case_box8_p3"),
+    (["--n", "8", "--porder", "3", "--gpu-geometry"], "case_box8_p3"),
 ])
 def test_fortran_driver(gpu, tmp_path, args, case):
     """The Fortran timer driver (drop-in elem_convec/elem_diffu modules over the
