@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import minisod2d_amd as m  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X, bytes/s
+FP64_PEAK = 78.6e12  # MI355X vector FP64, flop/s
 CALIB_BYTES = 2 << 30  # device-to-device copy that scales the HBM byte counters
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_convec_diffu_128cubed_p4.md")
 TRAFFIC_TAG = "traffic-json: "
@@ -140,6 +141,13 @@ def load_traffic(n, porder, kernel):
     return None
 
 
+def minimal_flops(E, q):
+    """Flops of the minimal formulation of the two residuals (per element
+    node: convection 252q + 506, diffusion 66q + 290), for reporting the
+    arithmetic side of the co-limitation next to the bytes."""
+    return E * (252 * q + 506), E * (66 * q + 290)
+
+
 TIMERS = {"fp64": (m.TIMER_CONVEC, m.TIMER_DIFFU),
           "mixed": (m.TIMER_CONVEC_MIXED, m.TIMER_DIFFU_MIXED),
           "geo32": (m.TIMER_CONVEC_GEO32, m.TIMER_DIFFU_GEO32)}
@@ -189,6 +197,14 @@ def timed(prob, R, D, steps, warmup, variant, f32=None):
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps,
                 convec_ms=c[0] / max(c[1], 1), diffu_ms=d[0] / max(d[1], 1),
                 zero_ms=zc[0] / max(zc[1], 1), launches_per_call=c[1] / max(steps, 1))
+
+
+def flops_block(E, q, r):
+    fc, fd = minimal_flops(E, q)
+    tc, td = r["convec_ms"] * 1e-3, r["diffu_ms"] * 1e-3
+    return {"per_pair_gflop": (fc + fd) / 1e9, "convec_tflops": fc / tc / 1e12,
+            "diffu_tflops": fd / td / 1e12, "pair_fp64_peak_frac": (fc + fd) / (tc + td) / FP64_PEAK,
+            "note": "minimal-formulation flops (252q+506, 66q+290 per element node) over kernel time"}
 
 
 def time_geometry(prob, threads):
@@ -392,6 +408,7 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)},
         "kernels_ms": {"convec": r64["convec_ms"], "diffu": r64["diffu_ms"],
                        "zero_fill": r64["zero_ms"]},
+        "flops": flops_block(E, p + 1, r64),
         "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,
         "parity_sampled_rel_l2": parity,
         "geometry": geometry,

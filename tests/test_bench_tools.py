@@ -83,3 +83,12 @@ def test_committed_traffic_summary_is_read_for_its_workload_only():
         assert rec["kernels"][k]["traffic_bytes"] >= rec["kernels"][k]["algorithmic_bytes"]
     assert bench.load_traffic(64, 4, "full_diffusion_ijk") is None
     assert bench.load_traffic(128, 3, "full_convec_ijk") is None
+
+
+def test_minimal_flops_128cubed_p4():
+    # 625 GFLOP per convection + diffusion pair at 128^3 porder 4
+    fc, fd = bench.minimal_flops(128 ** 3 * 125, 5)
+    assert abs((fc + fd) / 1e9 - 625.48) < 0.01
+    blk = bench.flops_block(128 ** 3 * 125, 5, {"convec_ms": 10.0, "diffu_ms": 10.0})
+    assert blk["convec_tflops"] == pytest.approx(fc / 1e-2 / 1e12)
+    assert blk["pair_fp64_peak_frac"] == pytest.approx((fc + fd) / 2e-2 / bench.FP64_PEAK)
