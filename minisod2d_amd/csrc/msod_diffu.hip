@@ -80,7 +80,7 @@ __device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64
 // the MFMA accumulators in VGPRs (about 148 in all) instead of 188 VGPRs and
 // 32 AGPRs, 3 waves per SIMD instead of 2 (4.7 against 6.2 ms at 96^3 p=3).
 template <int P, typename TG, typename TF, bool CLOSURE>
-__global__ void __launch_bounds__(Shape<P>::NT, CLOSURE ? 3 : 1) k_diffu(DiffuArgs<TG, TF> a, const float *mlp)
+__global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ? 3 : 1) k_diffu(DiffuArgs<TG, TF> a, const float *mlp)
 {
     using S = Shape<P>;
     using T = double;  // arithmetic
