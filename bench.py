@@ -243,6 +243,25 @@ def time_geometry(prob, threads):
             "note": "elem_jacobian: He, gpvol from coord on the GPU vs the host Fortran routine"}
 
 
+def time_generator(n, p, jitter, host_s, threads):
+    """Mesh, He/gpvol and fields of the workload generated on the GPU
+    (device_box_problem), against the host generator's wall time."""
+    m.timing_reset()
+    m.timing_enable(True)
+    m.synchronize()
+    t0 = time.perf_counter()
+    d = m.device_box_problem(n, p, jitter=jitter)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    m.timing_enable(False)
+    gen_ms, _ = m.timing_get(m.TIMER_GENERATOR)
+    geo_ms, _ = m.timing_get(m.TIMER_GEOMETRY)
+    m.delete(*d.device_inputs(), d.coord)
+    return {"device_kernels_ms": gen_ms + geo_ms, "device_wall_s": wall, "host_s": host_s,
+            "host_threads": threads,
+            "note": "box mesh, He/gpvol and fields: device kernels vs the host generator"}
+
+
 def sampled_parity(prob, out, nsample=3000, seed=7):
     """Residuals at randomly chosen nodes against the oracle run on exactly
     the elements that touch those nodes (each sampled node then receives all
@@ -339,8 +358,9 @@ def main():
     log(f"generating {n}^3 porder {p} (host, {args.threads} threads)")
     t0 = time.time()
     prob = m.box_problem(n, p, jitter=args.jitter, threads=min(args.threads, os.cpu_count() or 1))
+    host_gen_s = time.time() - t0
     N, E = prob.npoin, prob.nelem * prob.tables.nnode
-    log(f"generated npoin={N} elem-nodes={E} in {time.time()-t0:.1f}s")
+    log(f"generated npoin={N} elem-nodes={E} in {host_gen_s:.1f}s")
 
     m.init(0)
     if args.calibrate_copy:
@@ -371,6 +391,10 @@ def main():
         log("timing the device geometry (elem_jacobian)")
         geometry = time_geometry(prob, min(args.threads, os.cpu_count() or 1))
         log(f"geometry {geometry}")
+        log("timing the device input generator")
+        geometry["generator"] = time_generator(n, p, args.jitter, host_gen_s,
+                                               min(args.threads, os.cpu_count() or 1))
+        log(f"generator {geometry['generator']}")
 
     split = {}
     if not args.no_mixed:

@@ -90,8 +90,9 @@ int msod_timing_enable(int enable);
 /* kernel: 0 zero-fill, 1 convec, 2 diffu, 3 convec mixed, 4 diffu mixed,
  *         5 diffu learned closure, 6 closure training kernels (features,
  *         predict, sensitivity, gradient, adam, loss), 7 convec geo32,
- *         8 diffu geo32, 9 geometry (msod_elem_jacobian).  Returns
- *         accumulated ms and launch count. */
+ *         8 diffu geo32, 9 geometry (msod_elem_jacobian), 10 input
+ *         generator (msod_box_mesh, msod_flow_fields,
+ *         msod_viscosity_fields).  Returns accumulated ms and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
 
@@ -110,6 +111,34 @@ int msod_timing_reset(void);
 int msod_elem_jacobian(int porder, int nelem, int npoin, const int32_t *connec, const double *coord,
                        const double *dNgp, const double *wgp, double *gpvol, double *He,
                        int64_t *nbad);
+
+/* ---- synthetic inputs on the device ---------------------------------------
+ * No counterpart in the reference, whose driver builds disjoint copies of one
+ * element (geometry/sources/mod_mesh.f90:11-65) with fixed initial
+ * conditions (main.f90:317-361).  These are the device forms of the
+ * generator of the benchmark configurations (the host library's box mesh,
+ * flow and viscosity fields): the same numbers, drawn from counter-based
+ * SplitMix64 streams, so they do not depend on who generates them.
+ *   msod_box_mesh: conforming nx*ny*nz box of the unit cube, order porder,
+ *     npoin = (nx p+1)(ny p+1)(nz p+1), nodes on the GLL points taken from
+ *     xgp(nnode,3) and gmshAtoI/J/K; element-interior nodes moved by up to
+ *     `jitter` of the local 1-D spacing; connec(nelem,nnode) 1-based,
+ *     elements and nodes numbered x fastest; coord(npoin,3).
+ *   msod_flow_fields: u, q(npoin,3), rho, pr, E, Tem, mu_fluid(npoin) from
+ *     coord (u a Taylor-Green field, rho = 1 + 0.05 N(0,1) times
+ *     1 + rho_jump for x > 0.5, pr = 1/gamma + 0.1 cos 4 pi x, q = rho u,
+ *     E = pr/(gamma-1) + rho|u|^2/2, Tem = pr/(rho Rgas), mu_fluid = 1e-3).
+ *   msod_viscosity_fields: mu_e ~ U[0,mue_max), mu_sgs ~ U[0,musgs_max) in
+ *     the reference layout (nelem,ngaus).
+ * Outputs are host-present (create) or device arrays; no host traffic. */
+int msod_box_mesh(int nx, int ny, int nz, int porder, double jitter, int64_t seed, const double *xgp,
+                  const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                  int32_t *connec, double *coord);
+int msod_flow_fields(int64_t npoin, const double *coord, double gamma, double Rgas, int64_t seed_rho,
+                     double rho_jump, double *u, double *q, double *rho, double *pr, double *E,
+                     double *Tem, double *mu_fluid);
+int msod_viscosity_fields(int64_t nelem, int ngaus, int64_t seed_mue, int64_t seed_musgs,
+                          double mue_max, double musgs_max, double *mu_e, double *mu_sgs);
 
 /* ---- residual kernels --------------------------------------------------- */
 

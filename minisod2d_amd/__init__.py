@@ -40,7 +40,8 @@ MSOD_ERR_MODEL = 8
 
 # timer ids of msod_timing_get
 (TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP,
- TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32, TIMER_GEOMETRY) = range(10)
+ TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32, TIMER_GEOMETRY,
+ TIMER_GENERATOR) = range(11)
 
 # the C-ABI entry points declared in include/msod2d.h (checked by the tests)
 ABI_SYMBOLS = (
@@ -54,7 +55,8 @@ ABI_SYMBOLS = (
     "msod_full_convec_ijk_geo32", "msod_full_diffusion_ijk_geo32",
     "msod_full_diffusion_ijk_closure", "msod_closure_features", "msod_closure_predict",
     "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
-    "msod_closure_loss", "msod_elem_jacobian",
+    "msod_closure_loss", "msod_elem_jacobian", "msod_box_mesh", "msod_flow_fields",
+    "msod_viscosity_fields",
 )
 
 
@@ -112,6 +114,10 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_closure_adam.argtypes = [_vp] * 4 + [_i, _d, _d, _d, _d]
         L.msod_closure_loss.argtypes = [_i] + [_vp] * 6
         L.msod_elem_jacobian.argtypes = [_i, _i, _i] + [_vp] * 7
+        L.msod_box_mesh.argtypes = [_i, _i, _i, _i, _d, ctypes.c_int64] + [_vp] * 6
+        L.msod_flow_fields.argtypes = [ctypes.c_int64, _vp, _d, _d, ctypes.c_int64, _d] + [_vp] * 7
+        L.msod_viscosity_fields.argtypes = [ctypes.c_int64, _i, ctypes.c_int64, ctypes.c_int64,
+                                            _d, _d, _vp, _vp]
         _hip = L
     return _hip
 
@@ -518,6 +524,44 @@ def box_problem(nx: int, porder: int, ny: int | None = None, nz: int | None = No
     return Problem(t, nelem, npoin, connec, coord, He, gpvol, u, q, rho, pr, E, Tem, muf, mu_e,
                    mu_sgs, meta=dict(nx=nx, ny=ny, nz=nz, porder=porder, jitter=jitter,
                                      rho_jump=rho_jump, seeds=list(seeds)))
+
+
+def device_box_problem(nx: int, porder: int, ny: int | None = None, nz: int | None = None,
+                       jitter: float = 0.0, rho_jump: float = 0.0,
+                       seeds=(SEED_RHO, SEED_MUE, SEED_MUSGS, SEED_JITTER)) -> "Problem":
+    """box_problem's mesh and fields generated on the GPU (msod_box_mesh,
+    msod_flow_fields, msod_viscosity_fields), then He and gpvol
+    (msod_elem_jacobian).  The arrays are allocated on the host (untouched,
+    no resident pages until read) and created on the device: call
+    update_host to read them, delete when done."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    t = hex_tables(porder)
+    nn = t.nnode
+    nelem = nx * ny * nz
+    npoin = (nx * porder + 1) * (ny * porder + 1) * (nz * porder + 1)
+    e = lambda shape, dt=np.float64: np.empty(shape, dtype=dt, order="F")
+    connec, coord = e((nelem, nn), np.int32), e((npoin, 3))
+    He, gpvol = e((3, 3, nn, nelem)), e((1, nn, nelem))
+    u, q = e((npoin, 3)), e((npoin, 3))
+    rho, pr, E, Tem, muf = (e(npoin) for _ in range(5))
+    mu_e, mu_sgs = e((nelem, nn)), e((nelem, nn))
+    arrays = (connec, coord, He, gpvol, u, q, rho, pr, E, Tem, muf, mu_e, mu_sgs)
+    create(*arrays)
+    L = hip_lib()
+    _check(L.msod_box_mesh(nx, ny, nz, porder, float(jitter), int(seeds[3]), *map(_ptr, (
+        t.xgp, t.AtoI, t.AtoJ, t.AtoK, connec, coord))))
+    nbad = elem_jacobian(porder, nelem, npoin, connec, coord, t.dNgp, t.wgp, gpvol, He)
+    if nbad:
+        delete(*arrays)
+        raise ValueError(f"{nbad} quadrature points with non-positive Jacobian")
+    _check(L.msod_flow_fields(npoin, _ptr(coord), GAMMA, RGAS, int(seeds[0]), float(rho_jump),
+                              *map(_ptr, (u, q, rho, pr, E, Tem, muf))))
+    _check(L.msod_viscosity_fields(nelem, nn, int(seeds[1]), int(seeds[2]), 1e-3, 1e-4,
+                                   _ptr(mu_e), _ptr(mu_sgs)))
+    return Problem(t, nelem, npoin, connec, coord, He, gpvol, u, q, rho, pr, E, Tem, muf, mu_e,
+                   mu_sgs, meta=dict(nx=nx, ny=ny, nz=nz, porder=porder, jitter=jitter,
+                                     rho_jump=rho_jump, seeds=list(seeds), device=True))
 
 
 def reference_problem(nelem: int, porder: int) -> Problem:

@@ -4,6 +4,7 @@
 // layouts derived once per mesh, table validation and the kernel calls.
 #include "msod2d.h"
 #include "msod_closure.h"
+#include "msod_gen.h"
 #include "msod_geom.h"
 #include "msod_kernels.h"
 #include "msod_mlp.h"
@@ -198,7 +199,7 @@ struct DerivedGeo {
     void *geo, *gvol;
 };
 
-constexpr int kNumTimers = 10;
+constexpr int kNumTimers = 11;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -1201,6 +1202,154 @@ int msod_elem_jacobian(int porder, int nelem, int npoin, const int32_t *connec, 
                     counts[0], npoin);
     if (nbad) *nbad = (int64_t)counts[1];
     return finish_call("geometry kernel");
+}
+
+int msod_box_mesh(int nx, int ny, int nz, int porder, double jitter, int64_t seed, const double *xgp,
+                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, int32_t *connec,
+                  double *coord)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (porder < 1 || porder > kGenMaxP)
+        return fail(MSOD_ERR_PORDER, "porder=%d: the generator supports 1..%d", porder, kGenMaxP);
+    if (nx < 1 || ny < 1 || nz < 1 || !(jitter >= 0.0 && jitter < 1.0))
+        return fail(MSOD_ERR_ARG, "box %d x %d x %d, jitter %g", nx, ny, nz, jitter);
+    const int q = porder + 1, nn = q * q * q;
+    BoxArgs b{};
+    b.nx = nx;
+    b.ny = ny;
+    b.nz = nz;
+    b.p = porder;
+    b.nn = nn;
+    b.nelem = (int64_t)nx * ny * nz;
+    b.n1x = (int64_t)nx * porder + 1;
+    b.n1y = (int64_t)ny * porder + 1;
+    b.npoin = b.n1x * b.n1y * ((int64_t)nz * porder + 1);
+    if (b.npoin >= (int64_t)1 << 31 || b.nelem * nn >= (int64_t)1 << 31)
+        return fail(MSOD_ERR_ARG, "mesh too large for the int32 index format of the reference");
+    b.h[0] = 1.0 / nx;
+    b.h[1] = 1.0 / ny;
+    b.h[2] = 1.0 / nz;
+    b.jitter = jitter;
+    b.seed = (uint64_t)seed;
+    // lattice positions from the SOD 1-D indices (1: xi=-1, 2: xi=+1, 3..q:
+    // interior ascending) and the 1-D points from xgp(nnode,3)
+    std::vector<char> t0, t1, t2, t3;
+    const char *px, *pi, *pj, *pk;
+    int st;
+    if ((st = host_view(xgp, sizeof(double) * nn * 3, t0, &px, "xgp"))) return st;
+    if ((st = host_view(AtoI, sizeof(int32_t) * nn, t1, &pi, "gmshAtoI"))) return st;
+    if ((st = host_view(AtoJ, sizeof(int32_t) * nn, t2, &pj, "gmshAtoJ"))) return st;
+    if ((st = host_view(AtoK, sizeof(int32_t) * nn, t3, &pk, "gmshAtoK"))) return st;
+    const int32_t *A[3] = {(const int32_t *)pi, (const int32_t *)pj, (const int32_t *)pk};
+    const double *X = (const double *)px;
+    std::vector<int8_t> pos(3 * nn);
+    std::vector<int> seen(q, 0);
+    for (int a = 0; a < nn; ++a)
+        for (int d = 0; d < 3; ++d) {
+            const int i = A[d][a];
+            if (i < 1 || i > q) return fail(MSOD_ERR_TABLE, "gmshAto%c(%d) = %d", "IJK"[d], a + 1, i);
+            const int ps = i == 1 ? 0 : (i == 2 ? porder : i - 2);
+            pos[3 * a + d] = (int8_t)ps;
+            const double x = X[a + (int64_t)nn * d];
+            if (seen[ps] && b.xi[ps] != x) return fail(MSOD_ERR_TABLE, "xgp inconsistent with gmshAtoI/J/K");
+            b.xi[ps] = x;
+            seen[ps] = 1;
+        }
+    for (int ps = 1; ps < porder; ++ps)
+        b.hloc[ps] = std::min(b.xi[ps + 1] - b.xi[ps], b.xi[ps] - b.xi[ps - 1]);
+    const void *dc, *dx;
+    uint64_t vx;
+    if ((st = resolve(connec, sizeof(int32_t) * b.nelem * nn, "connec", &dc, &vx))) return st;
+    if ((st = resolve(coord, sizeof(double) * b.npoin * 3, "coord", &dx, &vx))) return st;
+    DevTemp dpos;
+    if ((st = dev_temp(pos, dpos, "generator tables"))) return st;
+    b.pos = (const int8_t *)dpos.p;
+    b.connec = (int32_t *)dc;
+    b.coord = (double *)dx;
+    TimedLaunch tk(10);
+    HIPCHK(launch_box_mesh(b, g.stream), "box mesh launch");
+    tk.done();
+    HIPCHK(hipStreamSynchronize(g.stream), "box mesh");  // the table is freed on return
+    for (const void *ptr : {(const void *)connec, (const void *)coord}) {
+        uintptr_t base;
+        if (Region *r = find_region(ptr, &base))
+            r->version = g.next_version++;
+        else
+            drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+    }
+    return finish_call("box mesh");
+}
+
+int msod_flow_fields(int64_t npoin, const double *coord, double gamma, double Rgas, int64_t seed_rho,
+                     double rho_jump, double *u, double *q, double *rho, double *pr, double *E,
+                     double *Tem, double *mu_fluid)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (npoin < 1 || !(gamma > 1.0) || !(Rgas > 0.0))
+        return fail(MSOD_ERR_ARG, "npoin=%lld gamma=%g Rgas=%g", (long long)npoin, gamma, Rgas);
+    FlowArgs f{};
+    f.npoin = npoin;
+    f.gamma = gamma;
+    f.Rgas = Rgas;
+    f.rho_jump = rho_jump;
+    f.seed_rho = (uint64_t)seed_rho;
+    const size_t n8 = sizeof(double) * (size_t)npoin;
+    const void *d[9];
+    uint64_t vx;
+    int st;
+    const void *in[9] = {coord, u, q, rho, pr, E, Tem, mu_fluid, nullptr};
+    const size_t sz[8] = {3 * n8, 3 * n8, 3 * n8, n8, n8, n8, n8, n8};
+    const char *nm[8] = {"coord", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid"};
+    for (int k = 0; k < 8; ++k)
+        if ((st = resolve(in[k], sz[k], nm[k], &d[k], &vx))) return st;
+    f.coord = (const double *)d[0];
+    f.u = (double *)d[1];
+    f.q = (double *)d[2];
+    f.rho = (double *)d[3];
+    f.pr = (double *)d[4];
+    f.E = (double *)d[5];
+    f.Tem = (double *)d[6];
+    f.mu_fluid = (double *)d[7];
+    TimedLaunch tk(10);
+    HIPCHK(launch_flow_fields(f, g.stream), "flow fields launch");
+    tk.done();
+    for (int k = 1; k < 8; ++k) {
+        uintptr_t base;
+        if (Region *r = find_region(in[k], &base)) r->version = g.next_version++;
+    }
+    return finish_call("flow fields");
+}
+
+int msod_viscosity_fields(int64_t nelem, int ngaus, int64_t seed_mue, int64_t seed_musgs, double mue_max,
+                          double musgs_max, double *mu_e, double *mu_sgs)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (nelem < 1 || ngaus < 1) return fail(MSOD_ERR_ARG, "nelem=%lld ngaus=%d", (long long)nelem, ngaus);
+    ViscArgs v{};
+    v.nelem = nelem;
+    v.ngaus = ngaus;
+    v.seed_mue = (uint64_t)seed_mue;
+    v.seed_musgs = (uint64_t)seed_musgs;
+    v.mue_max = mue_max;
+    v.musgs_max = musgs_max;
+    const void *d0, *d1;
+    uint64_t vx;
+    int st;
+    if ((st = resolve(mu_e, sizeof(double) * nelem * ngaus, "mu_e", &d0, &vx))) return st;
+    if ((st = resolve(mu_sgs, sizeof(double) * nelem * ngaus, "mu_sgs", &d1, &vx))) return st;
+    v.mu_e = (double *)d0;
+    v.mu_sgs = (double *)d1;
+    TimedLaunch tk(10);
+    HIPCHK(launch_viscosity_fields(v, g.stream), "viscosity fields launch");
+    tk.done();
+    for (const void *ptr : {(const void *)mu_e, (const void *)mu_sgs}) {
+        uintptr_t base;
+        if (Region *r = find_region(ptr, &base)) r->version = g.next_version++;
+    }
+    return finish_call("viscosity fields");
 }
 
 int msod_set_diffusion_coefficients(double c_rho, double c_ener)
