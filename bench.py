@@ -421,6 +421,7 @@ def main():
         "unit": "GLL-node updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": r64["ms_per_step"],
+        "elem_node_updates_per_s": E / (r64["ms_per_step"] * 1e-3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"unit cube {n}x{n}x{n} hex elements, porder {p}, "
