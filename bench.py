@@ -262,6 +262,78 @@ def time_generator(n, p, jitter, host_s, threads):
             "note": "box mesh, He/gpvol and fields: device kernels vs the host generator"}
 
 
+def tet_bytes(nelem):
+    """Algorithmic bytes of fem_convec and fem_diffu on a disjoint mesh of
+    nelem tetrahedra (4 nodes each, every input read once, outputs written
+    once): per element connec 16, He 288, gpvol 32 (+ mu_e, mu_sgs 64 in the
+    diffusion); per node u,q,rho,pr,E 72 (rho,u,Tem,mu_fluid 48) and 40 of
+    outputs."""
+    return nelem * (336 + 4 * (72 + 40)), nelem * (400 + 4 * (48 + 40))
+
+
+def time_tet(prob, R, D, steps, warmup, nsample=2000):
+    """fem_convec + fem_diffu on the reference driver's TET04 data over the
+    workload's nodal fields (npoin/4 disjoint tetrahedra, tet_jacobian on the
+    device), outputs in the workload's R, D; sampled elements against the oracle."""
+    from oracle_lib import Oracle, rel_l2
+    tp = m.tet_problem(prob, jitter=0.2, random_mu=True, strict=False)
+    t = tp.tables
+    He, gp = np.empty(tp.He.shape, order="F"), np.empty(tp.gpvol.shape, order="F")
+    els_in = (tp.connec, tp.coord, tp.mu_e, tp.mu_sgs)
+    m.copyin(*els_in)
+    m.create(He, gp)
+    try:
+        m.timing_reset()
+        m.timing_enable(True)
+        nbad = m.tet_jacobian(tp.nelem, tp.npoin, tp.connec, tp.coord, t.dNgp, t.wgp, gp, He)
+        m.timing_enable(False)
+        jac_ms, _ = m.timing_get(m.TIMER_TET_GEOMETRY)
+        tp.He, tp.gpvol = He, gp  # the device-computed geometry, as the reference driver uses
+        m.set_async(True)
+        for _ in range(warmup):
+            m.fem_convec(*tp.convec_args(*R))
+            m.fem_diffu(*tp.diffu_args(*D))
+        m.synchronize()
+        m.timing_reset()
+        m.timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.fem_convec(*tp.convec_args(*R))
+            m.fem_diffu(*tp.diffu_args(*D))
+        m.synchronize()
+        t1 = time.perf_counter()
+        m.timing_enable(False)
+        m.set_async(False)
+        c, d, z = (m.timing_get(k) for k in (m.TIMER_FEM_CONVEC, m.TIMER_FEM_DIFFU, m.TIMER_ZERO))
+        m.update_host(*R, *D, He, gp)
+    finally:
+        m.delete(*els_in, He, gp)
+    cms, dms, zms = c[0] / max(c[1], 1), d[0] / max(d[1], 1), z[0] / max(z[1], 1)
+    bc, bd = tet_bytes(tp.nelem)
+    rng = np.random.default_rng(9)
+    els = np.sort(rng.choice(tp.nelem, size=min(nsample, tp.nelem), replace=False))
+    nodes = (tp.connec[els] - 1).ravel()
+    f = np.asfortranarray
+    sub = m.TetProblem(t, len(els), 4 * len(els),
+                       f(np.arange(1, 4 * len(els) + 1, dtype=np.int32).reshape(len(els), 4)),
+                       f(tp.coord[nodes]), f(He[:, :, :, els]), f(gp[:, :, els]),
+                       *[f(getattr(tp, k)[nodes]) for k in ("u", "q", "rho", "pr", "E", "Tem", "mu_fluid")],
+                       f(tp.mu_e[els]), f(tp.mu_sgs[els]), Cp=tp.Cp, Pr=tp.Pr)
+    o = Oracle()
+    want = o.fem_convec(sub) + o.fem_diffu(sub)
+    errs = {k: rel_l2(g[nodes], w) for k, g, w in
+            zip(("Rmass", "Rmom", "Rener", "Dmass", "Dmom", "Dener"), R + D, want)}
+    return {"nelem": tp.nelem, "npoin": tp.npoin, "nbad": nbad, "tet_jacobian_ms": jac_ms,
+            "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "kernels_ms": {"fem_convec": cms, "fem_diffu": dms, "zero_fill": zms},
+            "bytes": {"fem_convec": bc, "fem_diffu": bd},
+            "hbm_frac": {"fem_convec": bc / (cms * 1e-3) / HBM_PEAK,
+                         "fem_diffu": bd / (dms * 1e-3) / HBM_PEAK},
+            "sampled_rel_l2": errs,
+            "note": "TET04 kernels on npoin/4 disjoint tetrahedra over the workload's nodal "
+                    "fields (main.f90:364-407), jittered vertices, random mu_e, mu_sgs"}
+
+
 def sampled_parity(prob, out, nsample=3000, seed=7):
     """Residuals at randomly chosen nodes against the oracle run on exactly
     the elements that touch those nodes (each sampled node then receives all
@@ -336,6 +408,7 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-geometry", action="store_true", help="skip the elem_jacobian timing")
+    ap.add_argument("--no-tet", action="store_true", help="skip the TET04 kernels")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--assembly", choices=("colored", "atomic"), default="atomic",
                     help="node assembly of the reported run")
@@ -396,6 +469,12 @@ def main():
                                                min(args.threads, os.cpu_count() or 1))
         log(f"generator {geometry['generator']}")
 
+    tet = None
+    if not args.no_tet:
+        log("timing the TET04 kernels")
+        tet = time_tet(prob, R, D, args.steps, args.warmup)
+        log(f"tet {tet}")
+
     split = {}
     if not args.no_mixed:
         f32 = prob.fp32()
@@ -437,6 +516,7 @@ def main():
         "pair_hbm_frac": (bc + bd) / ((r64["convec_ms"] + r64["diffu_ms"] + 2 * r64["zero_ms"]) * 1e-3) / HBM_PEAK,
         "parity_sampled_rel_l2": parity,
         "geometry": geometry,
+        "tet": tet,
         "assembly": args.assembly,
     }
     if other is not None:

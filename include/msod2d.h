@@ -92,7 +92,9 @@ int msod_timing_enable(int enable);
  *         predict, sensitivity, gradient, adam, loss), 7 convec geo32,
  *         8 diffu geo32, 9 geometry (msod_elem_jacobian), 10 input
  *         generator (msod_box_mesh, msod_flow_fields,
- *         msod_viscosity_fields).  Returns accumulated ms and launch count. */
+ *         msod_viscosity_fields), 11 TET convec (msod_fem_convec),
+ *         12 TET diffu (msod_fem_diffu), 13 TET geometry
+ *         (msod_tet_jacobian).  Returns accumulated ms and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
 
@@ -262,6 +264,38 @@ int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, 
  * momentum and energy residuals; target, rerr: [4][npoin]; weights[4] host */
 int msod_closure_loss(int npoin, const double *Rmom, const double *Rener, const double *target,
                       const double *weights, double *rerr, double *loss);
+
+/* ---- linear tetrahedra (TET04) ---------------------------------------------
+ * The reference driver times two P1-tetrahedron kernels beside the hex ones
+ * (main.f90:464-491), on a disjoint mesh of npoin/4 tetrahedra built over the
+ * same nodal fields (main.f90:367-407).  Layouts: connec(nelem,4) int32,
+ * 1-based; Ngp(4,4) (point, node); dNgp(3,4,4); wgp(4); coord(npoin,3);
+ * He(3,3,4,nelem); gpvol(1,4,nelem); mu_e, mu_sgs(nelem,4).  The element
+ * contributions are added with FP64 atomics into zeroed outputs, which is
+ * correct on any mesh (the reference's fem_diffu adds without atomics,
+ * elem_diffu.f90:306-312, relying on the mesh being disjoint).  A connec entry
+ * outside 1..npoin returns MSOD_ERR_MESH (checked on the device once per
+ * array version).
+ *
+ * tet_jacobian (src/fem/sources/jacobian_oper.f90:138-267): He and gpvol from
+ * coord, in the reference's operation order; *nbad = number of points with
+ * det(Je) <= 0. */
+int msod_tet_jacobian(int nelem, int npoin, const int32_t *connec, const double *coord,
+                      const double *dNgp, const double *wgp, double *gpvol, double *He,
+                      int64_t *nbad);
+/* fem_convec (src/kernels/sources/elem_convec.f90:839-948) */
+int msod_fem_convec(int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                    const double *dNgp, const double *He, const double *gpvol, const double *u,
+                    const double *q, const double *rho, const double *pr, const double *E,
+                    double *Rmass, double *Rmom, double *Rener);
+/* fem_diffu (src/kernels/sources/elem_diffu.f90:199-317); c_rho and c_ener as
+ * set by msod_set_diffusion_coefficients.  Like the reference, the fluid
+ * properties at Gauss point g weight node n by Ngp(n,g). */
+int msod_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                   const double *dNgp, const double *He, const double *gpvol, double Cp, double Pr,
+                   const double *rho, const double *u, const double *Tem, const double *mu_fluid,
+                   const double *mu_e, const double *mu_sgs, double *Rmass, double *Rmom,
+                   double *Rener);
 
 #ifdef __cplusplus
 }

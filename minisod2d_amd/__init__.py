@@ -41,7 +41,7 @@ MSOD_ERR_MODEL = 8
 # timer ids of msod_timing_get
 (TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP,
  TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32, TIMER_GEOMETRY,
- TIMER_GENERATOR) = range(11)
+ TIMER_GENERATOR, TIMER_FEM_CONVEC, TIMER_FEM_DIFFU, TIMER_TET_GEOMETRY) = range(14)
 
 # the C-ABI entry points declared in include/msod2d.h (checked by the tests)
 ABI_SYMBOLS = (
@@ -56,7 +56,7 @@ ABI_SYMBOLS = (
     "msod_full_diffusion_ijk_closure", "msod_closure_features", "msod_closure_predict",
     "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
     "msod_closure_loss", "msod_elem_jacobian", "msod_box_mesh", "msod_flow_fields",
-    "msod_viscosity_fields",
+    "msod_viscosity_fields", "msod_tet_jacobian", "msod_fem_convec", "msod_fem_diffu",
 )
 
 
@@ -118,6 +118,9 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_flow_fields.argtypes = [ctypes.c_int64, _vp, _d, _d, ctypes.c_int64, _d] + [_vp] * 7
         L.msod_viscosity_fields.argtypes = [ctypes.c_int64, _i, ctypes.c_int64, ctypes.c_int64,
                                             _d, _d, _vp, _vp]
+        L.msod_tet_jacobian.argtypes = [_i, _i] + [_vp] * 7
+        L.msod_fem_convec.argtypes = [_i, _i] + [_vp] * 13
+        L.msod_fem_diffu.argtypes = [_i, _i] + [_vp] * 5 + [_d, _d] + [_vp] * 9
         _hip = L
     return _hip
 
@@ -134,6 +137,7 @@ def host_lib() -> ctypes.CDLL:
         L.msodh_viscosity_fields.argtypes = [ctypes.c_int64, _i, ctypes.c_int64, ctypes.c_int64,
                                              _d, _d, _vp, _vp]
         L.msodh_disjoint_mesh.argtypes = [_i, _i, _i, _vp, _vp, _vp]
+        L.msodh_tet_tables.argtypes = [_vp] * 4
         L.msodh_interp_1d.argtypes = [_i, _i, _vp]
         L.msodh_prolong_fields.argtypes = [_i] * 6 + [_vp] * 4
         L.msodh_restrict_residual.argtypes = [_i] * 6 + [_vp] * 4
@@ -248,6 +252,31 @@ def elem_jacobian(porder, nelem, npoin, connec, coord, dNgp, wgp, gpvol, He) -> 
                                         *map(_ptr, (connec, coord, dNgp, wgp, gpvol, He)),
                                         ctypes.byref(nbad)))
     return int(nbad.value)
+
+
+def tet_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He) -> int:
+    """tet_jacobian (jacobian_oper.f90:138-267) on the GPU: He(3,3,4,nelem) and
+    gpvol(1,4,nelem) of linear tetrahedra from coord(npoin,3); returns the
+    number of points with det(Je) <= 0."""
+    nbad = ctypes.c_int64(0)
+    _check(hip_lib().msod_tet_jacobian(int(nelem), int(npoin),
+                                       *map(_ptr, (connec, coord, dNgp, wgp, gpvol, He)),
+                                       ctypes.byref(nbad)))
+    return int(nbad.value)
+
+
+def fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, Rmass, Rmom, Rener):
+    """Convective residual on linear tetrahedra (elem_convec.f90:839-948), FP64, on the GPU."""
+    _check(hip_lib().msod_fem_convec(int(nelem), int(npoin), *map(_ptr, (
+        connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, Rmass, Rmom, Rener))))
+
+
+def fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
+              mu_sgs, Rmass, Rmom, Rener):
+    """Diffusive residual on linear tetrahedra (elem_diffu.f90:199-317), FP64, on the GPU."""
+    _check(hip_lib().msod_fem_diffu(int(nelem), int(npoin), *map(_ptr, (
+        connec, Ngp, dNgp, He, gpvol)), float(Cp), float(Pr), *map(_ptr, (
+            rho, u, Tem, mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener))))
 
 
 # ---------------------------------------------------------------------------
@@ -562,6 +591,107 @@ def device_box_problem(nx: int, porder: int, ny: int | None = None, nz: int | No
     return Problem(t, nelem, npoin, connec, coord, He, gpvol, u, q, rho, pr, E, Tem, muf, mu_e,
                    mu_sgs, meta=dict(nx=nx, ny=ny, nz=nz, porder=porder, jitter=jitter,
                                      rho_jump=rho_jump, seeds=list(seeds), device=True))
+
+
+@dataclass
+class TetTables:
+    xgp: np.ndarray   # (4,3)
+    wgp: np.ndarray   # (4,)
+    Ngp: np.ndarray   # (4,4): (point, node)
+    dNgp: np.ndarray  # (3,4,4)
+
+
+def tet_tables() -> TetTables:
+    """The TET04 element of the reference driver (main.f90:374-383)."""
+    t = [_f((4, 3)), _f(4), _f((4, 4)), _f((3, 4, 4))]
+    host_lib().msodh_tet_tables(*[x.ctypes.data_as(_vp) for x in t])
+    return TetTables(*t)
+
+
+TET_BASE = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+SEED_TET = 1005
+
+
+@dataclass
+class TetProblem:
+    """Inputs of fem_convec / fem_diffu: the reference driver's disjoint
+    tetrahedral mesh (main.f90:367-407) over the nodal fields of a hex problem."""
+    tables: TetTables
+    nelem: int
+    npoin: int
+    connec: np.ndarray
+    coord: np.ndarray
+    He: np.ndarray
+    gpvol: np.ndarray
+    u: np.ndarray
+    q: np.ndarray
+    rho: np.ndarray
+    pr: np.ndarray
+    E: np.ndarray
+    Tem: np.ndarray
+    mu_fluid: np.ndarray
+    mu_e: np.ndarray
+    mu_sgs: np.ndarray
+    Cp: float = CP
+    Pr: float = PR
+    meta: dict = field(default_factory=dict)
+
+    def convec_args(self, Rmass, Rmom, Rener):
+        t = self.tables
+        return (self.nelem, self.npoin, self.connec, t.Ngp, t.dNgp, self.He, self.gpvol, self.u,
+                self.q, self.rho, self.pr, self.E, Rmass, Rmom, Rener)
+
+    def diffu_args(self, Rmass, Rmom, Rener):
+        t = self.tables
+        return (self.nelem, self.npoin, self.connec, t.Ngp, t.dNgp, self.He, self.gpvol, self.Cp,
+                self.Pr, self.rho, self.u, self.Tem, self.mu_fluid, self.mu_e, self.mu_sgs, Rmass,
+                Rmom, Rener)
+
+    def outputs(self):
+        return _f(self.npoin), _f((self.npoin, 3)), _f(self.npoin)
+
+    def device_inputs(self):
+        return (self.connec, self.He, self.gpvol, self.u, self.q, self.rho, self.pr, self.E,
+                self.Tem, self.mu_fluid, self.mu_e, self.mu_sgs)
+
+
+def tet_problem(prob: Problem, jitter: float = 0.0, random_mu: bool = False,
+                seed: int = SEED_TET, strict: bool = True) -> TetProblem:
+    """The reference driver's TET04 data over prob's nodal fields: nelem =
+    npoin/4 disjoint unit tetrahedra, each shifted by 3 along the diagonal
+    (mod_mesh.f90:71-125), He and gpvol from tet_jacobian, mu_e = mu_sgs = 1
+    (main.f90:403-407).  The reference stops unless 4 divides npoin; with
+    strict=False the last npoin mod 4 nodes are left out of the mesh instead.
+    jitter moves every vertex by up to jitter/2 per axis (seeded), so that He
+    is not the identity; random_mu draws mu_e, mu_sgs in [0,1) instead of ones."""
+    npoin = prob.npoin
+    if npoin % 4 and strict:
+        raise ValueError(f"npoin={npoin} is not divisible by 4 (main.f90:369-372)")
+    nelem = npoin // 4
+    H = host_lib()
+    t = tet_tables()
+    connec = _f((nelem, 4), np.int32)
+    coord = _f((npoin, 3))
+    H.msodh_disjoint_mesh(nelem, npoin, 4, np.asfortranarray(TET_BASE).ctypes.data,
+                          connec.ctypes.data, coord.ctypes.data)
+    rng = np.random.default_rng(seed)
+    if jitter:
+        coord[:4 * nelem] += jitter * (rng.random((4 * nelem, 3)) - 0.5)
+    He = _f((3, 3, 4, nelem))
+    gpvol = _f((1, 4, nelem))
+    nbad = H.msodh_elem_jacobian(nelem, npoin, 4, connec.ctypes.data, coord.ctypes.data,
+                                 t.dNgp.ctypes.data, t.wgp.ctypes.data, gpvol.ctypes.data,
+                                 He.ctypes.data)
+    if nbad:
+        raise ValueError(f"{nbad} tetrahedron points with non-positive Jacobian")
+    if random_mu:
+        mu_e, mu_sgs = np.asfortranarray(rng.random((nelem, 4))), np.asfortranarray(rng.random((nelem, 4)))
+    else:
+        mu_e, mu_sgs = np.ones((nelem, 4), order="F"), np.ones((nelem, 4), order="F")
+    return TetProblem(t, nelem, npoin, connec, coord, He, gpvol, prob.u, prob.q, prob.rho, prob.pr,
+                      prob.E, prob.Tem, prob.mu_fluid, mu_e, mu_sgs, Cp=prob.Cp, Pr=prob.Pr,
+                      meta=dict(prob.meta, tet=True, tet_jitter=jitter, tet_random_mu=random_mu,
+                                tet_seed=seed))
 
 
 def reference_problem(nelem: int, porder: int) -> Problem:

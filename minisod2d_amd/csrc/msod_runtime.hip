@@ -9,6 +9,7 @@
 #include "msod_kernels.h"
 #include "msod_mlp.h"
 #include "msod_plan.h"
+#include "msod_tet.h"
 
 #include <algorithm>
 #include <cmath>
@@ -199,7 +200,14 @@ struct DerivedGeo {
     void *geo, *gvol;
 };
 
-constexpr int kNumTimers = 11;
+// A tetrahedral connectivity found valid (every entry in 1..npoin).
+struct CheckedMesh {
+    const void *src;
+    uint64_t ver;
+    int64_t n, npoin;
+};
+
+constexpr int kNumTimers = 14;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -217,6 +225,7 @@ struct State {
     std::vector<TableSet *> tables;
     std::vector<PlanEntry> plans;
     std::vector<DerivedGeo> geos;
+    std::vector<CheckedMesh> checked;
     int assembly = kAtomic;
     double ms[kNumTimers] = {0};
     int64_t launches[kNumTimers] = {0};
@@ -505,6 +514,8 @@ void drop_derived_in(uintptr_t lo, uintptr_t hi)
             g.plans[i].release();
             g.plans.erase(g.plans.begin() + i);
         }
+    for (size_t i = g.checked.size(); i-- > 0;)
+        if (inside(g.checked[i].src)) g.checked.erase(g.checked.begin() + i);
     for (size_t i = g.geos.size(); i-- > 0;)
         if (inside(g.geos[i].srcHe) || inside(g.geos[i].srcVol)) {
             (void)hipFree(g.geos[i].geo);
@@ -1350,6 +1361,217 @@ int msod_viscosity_fields(int64_t nelem, int ngaus, int64_t seed_mue, int64_t se
         if (Region *r = find_region(ptr, &base)) r->version = g.next_version++;
     }
     return finish_call("viscosity fields");
+}
+
+// ---- linear tetrahedra ------------------------------------------------------
+extern "C++" {
+// Ngp(4,4), dNgp(3,4,4), wgp(4) of the TET04 element, read on the host
+static int tet_tables(const double *Ngp, const double *dNgp, const double *wgp, TetTables &t)
+{
+    std::vector<char> t0, t1, t2;
+    const char *pn = nullptr, *pd = nullptr, *pw = nullptr;
+    int st;
+    if (Ngp && (st = host_view(Ngp, sizeof t.Ngp, t0, &pn, "Ngp"))) return st;
+    if (dNgp && (st = host_view(dNgp, sizeof t.dNgp, t1, &pd, "dNgp"))) return st;
+    if (wgp && (st = host_view(wgp, sizeof t.wgp, t2, &pw, "wgp"))) return st;
+    std::memset(&t, 0, sizeof t);
+    if (pn) std::memcpy(t.Ngp, pn, sizeof t.Ngp);
+    if (pd) std::memcpy(t.dNgp, pd, sizeof t.dNgp);
+    if (pw) std::memcpy(t.wgp, pw, sizeof t.wgp);
+    for (double v : t.Ngp)
+        if (!std::isfinite(v)) return fail(MSOD_ERR_TABLE, "Ngp is not finite");
+    for (double v : t.dNgp)
+        if (!std::isfinite(v)) return fail(MSOD_ERR_TABLE, "dNgp is not finite");
+    return MSOD_OK;
+}
+
+// connec(nelem,4) entries in 1..npoin, checked on the device once per array
+// version (the kernels clamp in any case, so they never read out of bounds)
+static int check_tet_connec(const void *dc, uint64_t ver, int64_t nelem, int64_t npoin)
+{
+    const int64_t n = 4 * nelem;
+    for (const CheckedMesh &c : g.checked)
+        if (c.src == dc && c.ver == ver && c.n == n && c.npoin == npoin) return MSOD_OK;
+    std::vector<unsigned long long> count(1, 0ull);
+    DevTemp dk;
+    int st;
+    if ((st = dev_temp(count, dk, "connectivity check"))) return st;
+    HIPCHK(launch_connec_check((const int32_t *)dc, n, npoin, (unsigned long long *)dk.p, g.stream),
+           "connectivity check launch");
+    HIPCHK(hipMemcpyAsync(count.data(), dk.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream),
+           "connectivity check");
+    HIPCHK(hipStreamSynchronize(g.stream), "connectivity check");
+    if (count[0])
+        return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%lld)", count[0],
+                    (long long)npoin);
+    if (g.checked.size() >= kMaxDerived) g.checked.erase(g.checked.begin());
+    g.checked.push_back({dc, ver, n, npoin});
+    return MSOD_OK;
+}
+
+static int check_tet_sizes(int nelem, int npoin)
+{
+    if (nelem <= 0 || npoin <= 0) return fail(MSOD_ERR_ARG, "nelem=%d npoin=%d", nelem, npoin);
+    return MSOD_OK;
+}
+}  // extern "C++"
+
+int msod_tet_jacobian(int nelem, int npoin, const int32_t *connec, const double *coord,
+                      const double *dNgp, const double *wgp, double *gpvol, double *He, int64_t *nbad)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    int st = check_tet_sizes(nelem, npoin);
+    if (st) return st;
+    const int64_t NE = nelem, NP = npoin;
+    const void *dc, *dx, *dv, *dh;
+    uint64_t vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * 4, "connec", &dc, &vx))) return st;
+    if ((st = resolve(coord, sizeof(double) * NP * 3, "coord", &dx, &vx))) return st;
+    if ((st = resolve(gpvol, sizeof(double) * NE * 4, "gpvol", &dv, &vx))) return st;
+    if ((st = resolve(He, sizeof(double) * NE * 36, "He", &dh, &vx))) return st;
+    TetJacobianArgs a;
+    if (!dNgp || !wgp) return fail(MSOD_ERR_ARG, "dNgp and wgp are required");
+    if ((st = tet_tables(nullptr, dNgp, wgp, a.t))) return st;
+    std::vector<unsigned long long> counts(2, 0ull);
+    DevTemp dk;
+    if ((st = dev_temp(counts, dk, "geometry counters"))) return st;
+    a.nelem = NE;
+    a.npoin = NP;
+    a.connec = (const int32_t *)dc;
+    a.coord = (const double *)dx;
+    a.He = (double *)dh;
+    a.gpvol = (double *)dv;
+    a.counts = (unsigned long long *)dk.p;
+    TimedLaunch tk(13);
+    HIPCHK(launch_tet_jacobian(a, g.stream), "TET geometry kernel launch");
+    tk.done();
+    HIPCHK(hipMemcpyAsync(counts.data(), dk.p, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost,
+                          g.stream),
+           "TET geometry counters");
+    HIPCHK(hipStreamSynchronize(g.stream), "TET geometry kernel");
+    for (const void *ptr : {(const void *)He, (const void *)gpvol}) {
+        uintptr_t base;
+        if (Region *r = find_region(ptr, &base))
+            r->version = g.next_version++;
+        else
+            drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+    }
+    if (counts[0])
+        return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%d)",
+                    counts[0], npoin);
+    if (nbad) *nbad = (int64_t)counts[1];
+    return finish_call("TET geometry kernel");
+}
+
+int msod_fem_convec(int nelem, int npoin, const int32_t *connec, const double *Ngp, const double *dNgp,
+                    const double *He, const double *gpvol, const double *u, const double *q,
+                    const double *rho, const double *pr, const double *E, double *Rmass, double *Rmom,
+                    double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    int st = check_tet_sizes(nelem, npoin);
+    if (st) return st;
+    const int64_t NE = nelem, NP = npoin;
+    TetConvecArgs a;
+    if (!Ngp || !dNgp) return fail(MSOD_ERR_ARG, "Ngp and dNgp are required");
+    if ((st = tet_tables(Ngp, dNgp, nullptr, a.t))) return st;
+    const void *dc, *dh, *dv, *du, *dq, *dr, *dp, *dE, *dRm, *dRmo, *dRe;
+    uint64_t vc, vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * 4, "connec", &dc, &vc))) return st;
+    if ((st = resolve(He, sizeof(double) * NE * 36, "He", &dh, &vx))) return st;
+    if ((st = resolve(gpvol, sizeof(double) * NE * 4, "gpvol", &dv, &vx))) return st;
+    if ((st = resolve(u, sizeof(double) * NP * 3, "u", &du, &vx))) return st;
+    if ((st = resolve(q, sizeof(double) * NP * 3, "q", &dq, &vx))) return st;
+    if ((st = resolve(rho, sizeof(double) * NP, "rho", &dr, &vx))) return st;
+    if ((st = resolve(pr, sizeof(double) * NP, "pr", &dp, &vx))) return st;
+    if ((st = resolve(E, sizeof(double) * NP, "E", &dE, &vx))) return st;
+    if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
+    if ((st = resolve(Rmom, sizeof(double) * NP * 3, "Rmom", &dRmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
+    if ((st = check_tet_connec(dc, vc, NE, NP))) return st;
+    a.nelem = NE;
+    a.npoin = NP;
+    a.connec = (const int32_t *)dc;
+    a.He = (const double *)dh;
+    a.gpvol = (const double *)dv;
+    a.u = (const double *)du;
+    a.q = (const double *)dq;
+    a.rho = (const double *)dr;
+    a.pr = (const double *)dp;
+    a.E = (const double *)dE;
+    a.Rmass = (double *)dRm;
+    a.Rmom = (double *)dRmo;
+    a.Rener = (double *)dRe;
+    {
+        TimedLaunch tz(0);
+        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
+        tz.done();
+    }
+    TimedLaunch tk(11);
+    HIPCHK(launch_fem_convec(a, g.stream), "TET convection kernel launch");
+    tk.done();
+    return finish_call("TET convection kernel");
+}
+
+int msod_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ngp, const double *dNgp,
+                   const double *He, const double *gpvol, double Cp, double Pr, const double *rho,
+                   const double *u, const double *Tem, const double *mu_fluid, const double *mu_e,
+                   const double *mu_sgs, double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    int st = check_tet_sizes(nelem, npoin);
+    if (st) return st;
+    if (!(Pr != 0.0) || !std::isfinite(Cp) || !std::isfinite(Pr))
+        return fail(MSOD_ERR_ARG, "Cp=%g Pr=%g", Cp, Pr);
+    const int64_t NE = nelem, NP = npoin;
+    TetDiffuArgs a;
+    if (!Ngp || !dNgp) return fail(MSOD_ERR_ARG, "Ngp and dNgp are required");
+    if ((st = tet_tables(Ngp, dNgp, nullptr, a.t))) return st;
+    const void *dc, *dh, *dv, *dr, *du, *dT, *dmf, *dme, *dms, *dRm, *dRmo, *dRe;
+    uint64_t vc, vx;
+    if ((st = resolve(connec, sizeof(int32_t) * NE * 4, "connec", &dc, &vc))) return st;
+    if ((st = resolve(He, sizeof(double) * NE * 36, "He", &dh, &vx))) return st;
+    if ((st = resolve(gpvol, sizeof(double) * NE * 4, "gpvol", &dv, &vx))) return st;
+    if ((st = resolve(rho, sizeof(double) * NP, "rho", &dr, &vx))) return st;
+    if ((st = resolve(u, sizeof(double) * NP * 3, "u", &du, &vx))) return st;
+    if ((st = resolve(Tem, sizeof(double) * NP, "Tem", &dT, &vx))) return st;
+    if ((st = resolve(mu_fluid, sizeof(double) * NP, "mu_fluid", &dmf, &vx))) return st;
+    if ((st = resolve(mu_e, sizeof(double) * NE * 4, "mu_e", &dme, &vx))) return st;
+    if ((st = resolve(mu_sgs, sizeof(double) * NE * 4, "mu_sgs", &dms, &vx))) return st;
+    if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
+    if ((st = resolve(Rmom, sizeof(double) * NP * 3, "Rmom", &dRmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
+    if ((st = check_tet_connec(dc, vc, NE, NP))) return st;
+    a.nelem = NE;
+    a.npoin = NP;
+    a.connec = (const int32_t *)dc;
+    a.He = (const double *)dh;
+    a.gpvol = (const double *)dv;
+    a.rho = (const double *)dr;
+    a.u = (const double *)du;
+    a.Tem = (const double *)dT;
+    a.mu_fluid = (const double *)dmf;
+    a.mu_e = (const double *)dme;
+    a.mu_sgs = (const double *)dms;
+    a.Cp = Cp;
+    a.Pr = Pr;
+    a.c_rho = g.c_rho;
+    a.c_ener = g.c_ener;
+    a.Rmass = (double *)dRm;
+    a.Rmom = (double *)dRmo;
+    a.Rener = (double *)dRe;
+    {
+        TimedLaunch tz(0);
+        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
+        tz.done();
+    }
+    TimedLaunch tk(12);
+    HIPCHK(launch_fem_diffu(a, g.stream), "TET diffusion kernel launch");
+    tk.done();
+    return finish_call("TET diffusion kernel");
 }
 
 int msod_set_diffusion_coefficients(double c_rho, double c_ener)

@@ -1,7 +1,7 @@
 ! Drop-in replacements of the reference's kernel modules:
-!   module elem_convec    (kernels/sources/elem_convec.f90:198-211)
-!   module elem_diffu     (kernels/sources/elem_diffu.f90:10-23)
-!   module jacobian_oper  (fem/sources/jacobian_oper.f90:7-19)
+!   module elem_convec    (kernels/sources/elem_convec.f90:198-211, fem_convec :839-854)
+!   module elem_diffu     (kernels/sources/elem_diffu.f90:10-23, fem_diffu :199-212)
+!   module jacobian_oper  (fem/sources/jacobian_oper.f90:7-19, tet_jacobian :138-151)
 ! Same module and procedure names and the same argument lists, so a caller
 ! written for the reference compiles unchanged; the bodies run the gfx950
 ! kernels through the C ABI.  Differences a caller can see:
@@ -17,13 +17,41 @@
 !     reference's split form (elem_convec.f90:694-697); the fused reference
 !     kernel leaves two of its flux arrays uninitialised (:218,:248-250).
 !   * shapes are checked; a mismatch stops with a message.
+! Shape check shared by the TET04 procedures below.
+module msod_tet_shapes
+
+   implicit none
+   private
+   public :: check_tet_shapes
+
+contains
+
+   ! connec(nelem,4), Ngp(4,4), dNgp(3,4,4), He(3,3,4,nelem), gpvol(1,4,nelem),
+   ! nodal arrays of at least npoin entries
+   subroutine check_tet_shapes(who, nelem, npoin, connec, Ngp, dNgp, He, gpvol, nu, nr1, nr2, nr3)
+      character(*), intent(in) :: who
+      integer, intent(in) :: nelem, npoin, nu, nr1, nr2, nr3
+      integer(4), intent(in) :: connec(:,:)
+      real(8), intent(in) :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), gpvol(:,:,:)
+      if (size(connec,1) /= nelem .or. size(connec,2) /= 4 .or. size(Ngp,1) /= 4 .or. size(Ngp,2) /= 4 .or. &
+          size(dNgp,1) /= 3 .or. size(dNgp,2) /= 4 .or. size(dNgp,3) /= 4 .or. size(He,3) /= 4 .or. &
+          size(He,4) < nelem .or. size(gpvol,2) /= 4 .or. size(gpvol,3) < nelem .or. &
+          nu < npoin .or. nr1 < npoin .or. nr2 < npoin .or. nr3 < npoin) then
+         write(*,'(a,a,a,i0,a,i0)') 'msod: ', who, ': array shapes do not match nelem=', nelem, ' npoin=', npoin
+         error stop 2
+      end if
+   end subroutine check_tet_shapes
+
+end module msod_tet_shapes
+
 module elem_convec
 
    use iso_c_binding
-   use msod_device, only: msod_full_convec_ijk, msod_check
+   use msod_device, only: msod_full_convec_ijk, msod_fem_convec, msod_check
+   use msod_tet_shapes, only: check_tet_shapes
    implicit none
    private
-   public :: full_convec_ijk
+   public :: full_convec_ijk, fem_convec
 
 contains
 
@@ -61,16 +89,33 @@ contains
       end if
    end subroutine check_shapes
 
+   ! Linear tetrahedra (elem_convec.f90:839-948); the contributions meet at the
+   ! nodes through FP64 atomics, as in the reference.
+   subroutine fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), gpvol(:,:,:)
+      real(8), intent(in), contiguous :: q(:,:), u(:,:), rho(:), pr(:), E(:)
+      real(8), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      call check_tet_shapes('fem_convec', nelem, npoin, connec, Ngp, dNgp, He, gpvol, size(u,1), &
+                            size(Rmass), size(Rmom,1), size(Rener))
+      call msod_check(msod_fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, &
+                      Rmass, Rmom, Rener), 'fem_convec')
+   end subroutine fem_convec
+
 end module elem_convec
+
+
 
 module elem_diffu
 
    use iso_c_binding
    use msod_device, only: msod_full_diffusion_ijk, msod_set_diffusion_coefficients, msod_check, &
-                          msod_full_diffusion_ijk_closure
+                          msod_full_diffusion_ijk_closure, msod_fem_diffu
+   use msod_tet_shapes, only: check_tet_shapes
    implicit none
    private
-   public :: full_diffusion_ijk, full_diffusion_ijk_closure, c_rho, c_ener
+   public :: full_diffusion_ijk, full_diffusion_ijk_closure, fem_diffu, c_rho, c_ener
 
    real(8) :: c_rho = 1.0d0
    real(8) :: c_ener = 1.0d0
@@ -139,15 +184,35 @@ contains
       used_model = used /= 0
    end subroutine full_diffusion_ijk_closure
 
+   ! Linear tetrahedra (elem_diffu.f90:199-317).
+   subroutine fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, &
+                        mu_sgs, Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), gpvol(:,:,:)
+      real(8), intent(in) :: Cp, Pr
+      real(8), intent(in), contiguous :: rho(:), u(:,:), Tem(:), mu_fluid(:), mu_e(:,:), mu_sgs(:,:)
+      real(8), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      call check_tet_shapes('fem_diffu', nelem, npoin, connec, Ngp, dNgp, He, gpvol, size(u,1), &
+                            size(Rmass), size(Rmom,1), size(Rener))
+      if (size(mu_e,1) /= nelem .or. size(mu_e,2) /= 4 .or. size(mu_sgs,1) /= nelem .or. size(mu_sgs,2) /= 4) then
+         write(*,'(a,i0)') 'msod: fem_diffu: mu_e, mu_sgs must be (nelem,4), nelem=', nelem
+         error stop 2
+      end if
+      call msod_check(msod_set_diffusion_coefficients(c_rho, c_ener), 'set_diffusion_coefficients')
+      call msod_check(msod_fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp, Pr, rho, u, Tem, &
+                      mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener), 'fem_diffu')
+   end subroutine fem_diffu
+
 end module elem_diffu
 
 module jacobian_oper
 
    use iso_c_binding
-   use msod_device, only: msod_jacobian_device, msod_check
+   use msod_device, only: msod_jacobian_device, msod_tet_jacobian_device, msod_check
    implicit none
    private
-   public :: elem_jacobian
+   public :: elem_jacobian, tet_jacobian
 
 contains
 
@@ -181,5 +246,27 @@ contains
          error stop 3
       end if
    end subroutine elem_jacobian
+
+   ! He(3,3,4,nelem) and gpvol(1,4,nelem) of linear tetrahedra on the GPU
+   ! (jacobian_oper.f90:138-267); arrays present on the device, as above.
+   subroutine tet_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(8), intent(in), contiguous :: coord(:,:), dNgp(:,:,:), wgp(:)
+      real(8), intent(out), contiguous :: gpvol(:,:,:), He(:,:,:,:)
+      integer(c_int64_t) :: nbad
+      if (size(connec,1) /= nelem .or. size(connec,2) /= 4 .or. size(coord,1) < npoin .or. &
+          size(dNgp,2) /= 4 .or. size(wgp) < 4 .or. size(He,3) /= 4 .or. size(He,4) < nelem .or. &
+          size(gpvol,2) /= 4 .or. size(gpvol,3) < nelem) then
+         write(*,'(a,i0,a,i0)') 'msod: tet_jacobian: array shapes do not match nelem=', nelem, ' npoin=', npoin
+         error stop 2
+      end if
+      call msod_check(msod_tet_jacobian_device(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He, nbad), &
+                      'tet_jacobian')
+      if (nbad > 0) then
+         write(*,'(a,i0,a)') 'msod: tet_jacobian: ', nbad, ' points with det(Je) <= 0'
+         error stop 3
+      end if
+   end subroutine tet_jacobian
 
 end module jacobian_oper

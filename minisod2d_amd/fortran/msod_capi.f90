@@ -6,7 +6,7 @@ module msod_capi
 
    use iso_c_binding
    use omp_lib, only: omp_set_num_threads, omp_get_max_threads
-   use msod_fem, only: msod_gll, msod_hex_tables, msod_gmsh_hex
+   use msod_fem, only: msod_gll, msod_hex_tables, msod_gmsh_hex, msod_tet_tables
    use msod_mesh, only: msod_box_mesh, msod_disjoint_mesh, msod_box_sizes
    use msod_geom, only: msod_elem_jacobian
    use msod_fields, only: msod_flow_fields, msod_viscosity_fields
@@ -50,6 +50,12 @@ contains
       real(c_double), intent(out) :: dlxigp_ip((p+1)**3,3,p+1)
       call msod_hex_tables(p, invAtoIJK, AtoIJK, AtoI, AtoJ, AtoK, xgp, wgp, Ngp, dNgp, dlxigp_ip)
    end subroutine msodh_hex_tables
+
+   ! TET04 tables: xgp(4,3), wgp(4), Ngp(4,4), dNgp(3,4,4).
+   subroutine msodh_tet_tables(xgp, wgp, Ngp, dNgp) bind(C, name='msodh_tet_tables')
+      real(c_double), intent(out) :: xgp(4,3), wgp(4), Ngp(4,4), dNgp(3,4,4)
+      call msod_tet_tables(xgp, wgp, Ngp, dNgp)
+   end subroutine msodh_tet_tables
 
    subroutine msodh_box_sizes(nx, ny, nz, p, nelem, npoin) bind(C, name='msodh_box_sizes')
       integer(c_int), value :: nx, ny, nz, p

@@ -20,6 +20,7 @@ module msod_device
    public :: msod_full_convec_ijk, msod_full_diffusion_ijk, msod_full_convec_ijk_mixed
    public :: msod_full_diffusion_ijk_mixed, msod_set_diffusion_coefficients
    public :: msod_full_diffusion_ijk_closure, msod_jacobian_device
+   public :: msod_tet_jacobian_device, msod_fem_convec, msod_fem_diffu
 
    interface
       integer(c_int) function msod_init(device) bind(C, name='msod_init')
@@ -138,6 +139,33 @@ module msod_device
          type(*), dimension(*) :: gpvol, He
          integer(c_int64_t), intent(out) :: nbad
       end function msod_jacobian_device
+
+      integer(c_int) function msod_tet_jacobian_device(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He, &
+            nbad) bind(C, name='msod_tet_jacobian')
+         import :: c_int, c_int64_t
+         integer(c_int), value :: nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, coord, dNgp, wgp
+         type(*), dimension(*) :: gpvol, He
+         integer(c_int64_t), intent(out) :: nbad
+      end function msod_tet_jacobian_device
+
+      integer(c_int) function msod_fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, &
+            Rmass, Rmom, Rener) bind(C, name='msod_fem_convec')
+         import :: c_int
+         integer(c_int), value :: nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E
+         type(*), dimension(*) :: Rmass, Rmom, Rener
+      end function msod_fem_convec
+
+      integer(c_int) function msod_fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp, Pr, rho, u, Tem, &
+            mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener) bind(C, name='msod_fem_diffu')
+         import :: c_int, c_double
+         integer(c_int), value :: nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, Ngp, dNgp, He, gpvol
+         real(c_double), value :: Cp, Pr
+         type(*), dimension(*), intent(in) :: rho, u, Tem, mu_fluid, mu_e, mu_sgs
+         type(*), dimension(*) :: Rmass, Rmom, Rener
+      end function msod_fem_diffu
    end interface
 
 contains

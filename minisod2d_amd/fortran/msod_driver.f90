@@ -1,14 +1,20 @@
 ! miniSOD timer driver on the gfx950 kernels: counterpart of the reference's
 ! program main (app/sources/main.f90).  Builds the element tables, a mesh
 ! with its geometry and fields, mirrors them on the GPU, times nruns calls of
-! full_convec_ijk and full_diffusion_ijk (each call synchronous, the
-! reference's timed region, main.f90:444-496), writes timers.dat in the
-! reference's format (main.f90:495,499; the two TET columns are written as 0)
-! and prints the reference's summary (main.f90:504-581) plus node-update rates.
+! full_convec_ijk, fem_convec, full_diffusion_ijk and fem_diffu (each call
+! synchronous, the reference's timed region, main.f90:444-496), writes
+! timers.dat in the reference's format (main.f90:495,499) and prints the
+! reference's summary (main.f90:504-581) plus node-update rates.
+!
+! The TET04 kernels run, as in the reference (main.f90:364-407), on npoin/4
+! disjoint tetrahedra over the same nodal fields, with mu_e = mu_sgs = 1 and
+! tet_jacobian on the device copies.  The reference stops when 4 does not
+! divide npoin; here the TET part is skipped then (its timers.dat columns are
+! 0), and also with --mixed or --no-tet.
 !
 ! Usage: miniSOD [--n N] [--nelem NE] [--porder P] [--nruns R] [--jitter J]
 !                [--rho-jump F] [--mixed] [--assembly colored|atomic] [--quiet]
-!                [--gpu-geometry]
+!                [--gpu-geometry] [--no-tet]
 !   --n N      conforming N x N x N box of the unit cube (default 8)
 !   --nelem NE the reference's own case instead: NE disjoint elements with
 !              its initial conditions (main.f90:257-413)
@@ -16,22 +22,23 @@
 !   --gpu-geometry  compute He and gpvol on the device (elem_jacobian of the
 !              drop-in module jacobian_oper) on the device copies, as the
 !              reference does (main.f90:280-295); FP64 kernels only
+!   --assembly atomic (default, the library's default) or colored
 program miniSOD
 
    use iso_c_binding
-   use msod_fem, only: msod_hex_tables
+   use msod_fem, only: msod_hex_tables, msod_tet_tables
    use msod_mesh, only: msod_box_mesh, msod_box_sizes, msod_disjoint_mesh
    use msod_geom, only: msod_elem_jacobian
    use msod_fields, only: msod_flow_fields, msod_viscosity_fields
    use msod_device
-   use elem_convec, only: full_convec_ijk
-   use elem_diffu, only: full_diffusion_ijk
-   use jacobian_oper, only: elem_jacobian
+   use elem_convec, only: full_convec_ijk, fem_convec
+   use elem_diffu, only: full_diffusion_ijk, fem_diffu
+   use jacobian_oper, only: elem_jacobian, tet_jacobian
    implicit none
 
-   integer :: n = 8, nelem_ref = 0, porder = 4, nruns = 10, nn, q, i, nbad, assembly = 0
+   integer :: n = 8, nelem_ref = 0, porder = 4, nruns = 10, nn, q, i, nbad, assembly = 1
    real(8) :: jitter = 0d0, rho_jump = 0d0
-   logical :: mixed = .false., quiet = .false., gpu_geometry = .false.
+   logical :: mixed = .false., quiet = .false., gpu_geometry = .false., tet = .true.
    integer(8) :: nelem8, npoin8
    integer :: nelem, npoin
    integer(c_int), allocatable :: connec(:,:), invAtoIJK(:,:,:), AtoIJK(:), AtoI(:), AtoJ(:), AtoK(:)
@@ -43,6 +50,15 @@ program miniSOD
    real(c_float), allocatable :: muf4(:), mue4(:,:), mus4(:,:)
    real(8), parameter :: gamma = 1.4d0, Rgas = 287d0
    real(8) :: Cp, Pra, tc, td, tavg_c, tavg_d, tmax_c, tmax_d, tmin_c, tmin_d
+   ! TET04 data (main.f90:364-407)
+   integer :: nelem_t
+   integer(c_int), allocatable :: connec_t(:,:)
+   real(c_double), allocatable :: coord_t(:,:), xgp_t(:,:), wgp_t(:), Ngp_t(:,:), dNgp_t(:,:,:)
+   real(c_double), allocatable :: He_t(:,:,:,:), gpvol_t(:,:,:), mu_e_t(:,:), mu_sgs_t(:,:)
+   real(c_double), allocatable :: Rmass_t(:), Rmom_t(:,:), Rener_t(:), Dmass_t(:), Dmom_t(:,:), Dener_t(:)
+   real(8) :: tct, tdt, tavg_ct, tavg_dt, tmax_ct, tmax_dt, tmin_ct, tmin_dt
+   real(c_double), parameter :: xyzTET(4,3) = reshape([0d0, 1d0, 0d0, 0d0, 0d0, 0d0, 1d0, 0d0, &
+                                                       0d0, 0d0, 0d0, 1d0], [4, 3])
    integer(8) :: c0, c1, c2, rate
 
    call parse_args()
@@ -146,38 +162,98 @@ program miniSOD
       call msod_copyin(mu_sgs, c_sizeof(mu_sgs))
    end if
 
+   ! ---- TET04 data (main.f90:364-407) ----------------------------------------------
+   if (mod(npoin, 4) /= 0 .or. mixed) tet = .false.
+   if (tet) then
+      nelem_t = npoin/4
+      allocate(xgp_t(4,3), wgp_t(4), Ngp_t(4,4), dNgp_t(3,4,4))
+      call msod_tet_tables(xgp_t, wgp_t, Ngp_t, dNgp_t)
+      allocate(connec_t(nelem_t,4), coord_t(npoin,3))
+      call msod_disjoint_mesh(nelem_t, npoin, 4, xyzTET, connec_t, coord_t)
+      allocate(He_t(3,3,4,nelem_t), gpvol_t(1,4,nelem_t), mu_e_t(nelem_t,4), mu_sgs_t(nelem_t,4))
+      mu_e_t = 1d0
+      mu_sgs_t = 1d0
+      allocate(Rmass_t(npoin), Rmom_t(npoin,3), Rener_t(npoin), Dmass_t(npoin), Dmom_t(npoin,3), Dener_t(npoin))
+      call msod_copyin(connec_t, c_sizeof(connec_t))
+      call msod_copyin(coord_t, c_sizeof(coord_t))
+      call msod_copyin(dNgp_t, c_sizeof(dNgp_t))
+      call msod_copyin(wgp_t, c_sizeof(wgp_t))
+      call msod_create(He_t, c_sizeof(He_t))
+      call msod_create(gpvol_t, c_sizeof(gpvol_t))
+      call tet_jacobian(nelem_t, npoin, connec_t, coord_t, dNgp_t, wgp_t, gpvol_t, He_t)
+      call msod_copyin(mu_e_t, c_sizeof(mu_e_t))
+      call msod_copyin(mu_sgs_t, c_sizeof(mu_sgs_t))
+      call msod_create(Rmass_t, c_sizeof(Rmass_t))
+      call msod_create(Rmom_t, c_sizeof(Rmom_t))
+      call msod_create(Rener_t, c_sizeof(Rener_t))
+      call msod_create(Dmass_t, c_sizeof(Dmass_t))
+      call msod_create(Dmom_t, c_sizeof(Dmom_t))
+      call msod_create(Dener_t, c_sizeof(Dener_t))
+   else
+      write(*,*) 'TET04 kernels skipped (npoin not divisible by 4, or --mixed / --no-tet)'
+   end if
+
    ! ---- timing loop (main.f90:427-499) ------------------------------------------
    open(unit=1, file='timers.dat', status='replace')
    tavg_c = 0d0; tavg_d = 0d0; tmax_c = 0d0; tmax_d = 0d0; tmin_c = huge(1d0); tmin_d = huge(1d0)
+   tavg_ct = 0d0; tavg_dt = 0d0; tmax_ct = 0d0; tmax_dt = 0d0; tmin_ct = huge(1d0); tmin_dt = huge(1d0)
+   tct = 0d0; tdt = 0d0
    call system_clock(count_rate=rate)
    do i = 1, nruns
       call system_clock(c0)
       call convec()
       call system_clock(c1)
+      tc = dble(c1 - c0)/dble(rate)
+      if (tet) then
+         call system_clock(c0)
+         call fem_convec(nelem_t, npoin, connec_t, Ngp_t, dNgp_t, He_t, gpvol_t, u, qm, rho, pr, E, &
+                         Rmass_t, Rmom_t, Rener_t)
+         call system_clock(c1)
+         tct = dble(c1 - c0)/dble(rate)
+      end if
+      call system_clock(c1)
       call diffu()
       call system_clock(c2)
-      tc = dble(c1 - c0)/dble(rate)
       td = dble(c2 - c1)/dble(rate)
+      if (tet) then
+         call system_clock(c1)
+         call fem_diffu(nelem_t, npoin, connec_t, Ngp_t, dNgp_t, He_t, gpvol_t, Cp, Pra, rho, u, Tem, &
+                        mu_fluid, mu_e_t, mu_sgs_t, Dmass_t, Dmom_t, Dener_t)
+         call system_clock(c2)
+         tdt = dble(c2 - c1)/dble(rate)
+      end if
       tavg_c = tavg_c + tc; tmax_c = max(tmax_c, tc); tmin_c = min(tmin_c, tc)
       tavg_d = tavg_d + td; tmax_d = max(tmax_d, td); tmin_d = min(tmin_d, td)
-      write(1,10) i, tc, 0d0, td, 0d0
+      tavg_ct = tavg_ct + tct; tmax_ct = max(tmax_ct, tct); tmin_ct = min(tmin_ct, tct)
+      tavg_dt = tavg_dt + tdt; tmax_dt = max(tmax_dt, tdt); tmin_dt = min(tmin_dt, tdt)
+      write(1,10) i, tc, tct, td, tdt
    end do
    close(1)
 10 format(i3,1x,4(f16.8,1x))
    tavg_c = tavg_c/dble(nruns)
    tavg_d = tavg_d/dble(nruns)
+   tavg_ct = tavg_ct/dble(nruns)
+   tavg_dt = tavg_dt/dble(nruns)
    write(*,*)
    write(*,*) 'Timings:'
    write(*,*) '----------------------------------------'
    write(*,*) 'Avg. convective time     = ', tavg_c
+   if (tet) write(*,*) 'Avg. TET convective time = ', tavg_ct
    write(*,*) 'Avg. diffusive time      = ', tavg_d
+   if (tet) write(*,*) 'Avg. TET diffusive time  = ', tavg_dt
    write(*,*) 'Max. convective time     = ', tmax_c
+   if (tet) write(*,*) 'Max. TET convective time = ', tmax_ct
    write(*,*) 'Max. diffusive time      = ', tmax_d
+   if (tet) write(*,*) 'Max. TET diffusive time  = ', tmax_dt
    write(*,*) 'Min. convective time     = ', tmin_c
+   if (tet) write(*,*) 'Min. TET convective time = ', tmin_ct
    write(*,*) 'Min. diffusive time      = ', tmin_d
+   if (tet) write(*,*) 'Min. TET diffusive time  = ', tmin_dt
    write(*,*) '----------------------------------------'
    write(*,*) 'Variation convec.        = ', (tmax_c - tmin_c)/tavg_c
+   if (tet) write(*,*) 'Variation TET convec.    = ', (tmax_ct - tmin_ct)/tavg_ct
    write(*,*) 'Variation diffu.         = ', (tmax_d - tmin_d)/tavg_d
+   if (tet) write(*,*) 'Variation TET diffu.     = ', (tmax_dt - tmin_dt)/tavg_dt
    write(*,*) 'GLL-node updates/s (min) = ', dble(npoin)/(tmin_c + tmin_d)
    write(*,*) '----------------------------------------'
 
@@ -205,6 +281,31 @@ program miniSOD
       write(*,*) 'Max Dmom(:,2) = ', maxval(Dmom(:,2)), 'Min Dmom(:,2) = ', minval(Dmom(:,2))
       write(*,*) 'Max Dmom(:,3) = ', maxval(Dmom(:,3)), 'Min Dmom(:,3) = ', minval(Dmom(:,3))
       write(*,*) 'Max Dener     = ', maxval(Dener), 'Min Dener     = ', minval(Dener)
+   end if
+   if (tet) then
+      call msod_update_host(Rmass_t, c_sizeof(Rmass_t))
+      call msod_update_host(Rmom_t, c_sizeof(Rmom_t))
+      call msod_update_host(Rener_t, c_sizeof(Rener_t))
+      call msod_update_host(Dmass_t, c_sizeof(Dmass_t))
+      call msod_update_host(Dmom_t, c_sizeof(Dmom_t))
+      call msod_update_host(Dener_t, c_sizeof(Dener_t))
+      if (.not. quiet) then
+         write(*,*) 'Basic TET results:'
+         write(*,*) '----------------------------------------'
+         write(*,*) '--| Convec'
+         write(*,*) 'Max Rmass     = ', maxval(Rmass_t), 'Min Rmass     = ', minval(Rmass_t)
+         write(*,*) 'Max Rmom(:,1) = ', maxval(Rmom_t(:,1)), 'Min Rmom(:,1) = ', minval(Rmom_t(:,1))
+         write(*,*) 'Max Rmom(:,2) = ', maxval(Rmom_t(:,2)), 'Min Rmom(:,2) = ', minval(Rmom_t(:,2))
+         write(*,*) 'Max Rmom(:,3) = ', maxval(Rmom_t(:,3)), 'Min Rmom(:,3) = ', minval(Rmom_t(:,3))
+         write(*,*) 'Max Rener     = ', maxval(Rener_t), 'Min Rener     = ', minval(Rener_t)
+         write(*,*) '----------------------------------------'
+         write(*,*) '--| Diffu'
+         write(*,*) 'Max Dmass     = ', maxval(Dmass_t), 'Min Dmass     = ', minval(Dmass_t)
+         write(*,*) 'Max Dmom(:,1) = ', maxval(Dmom_t(:,1)), 'Min Dmom(:,1) = ', minval(Dmom_t(:,1))
+         write(*,*) 'Max Dmom(:,2) = ', maxval(Dmom_t(:,2)), 'Min Dmom(:,2) = ', minval(Dmom_t(:,2))
+         write(*,*) 'Max Dmom(:,3) = ', maxval(Dmom_t(:,3)), 'Min Dmom(:,3) = ', minval(Dmom_t(:,3))
+         write(*,*) 'Max Dener     = ', maxval(Dener_t), 'Min Dener     = ', minval(Dener_t)
+      end if
    end if
    call write_results()
    call msod_stop()
@@ -291,6 +392,8 @@ contains
             quiet = .true.
          case ('--gpu-geometry')
             gpu_geometry = .true.
+         case ('--no-tet')
+            tet = .false.
          case ('--n', '--nelem', '--porder', '--nruns', '--jitter', '--rho-jump', '--assembly')
             if (k == nargs) stop 'missing value'
             call get_command_argument(k+1, v)

@@ -23,6 +23,7 @@ module msod_fem
 
    public :: msod_gll, msod_sod_order, msod_lagrange, msod_lagrange_deriv
    public :: msod_gmsh_hex, msod_hex_tables, msod_deriv_matrix
+   public :: msod_tet_tables
 
    ! Corner pairs of the 12 hex edges and corner quadruples of the 6 faces,
    ! in Gmsh numbering (the same tables as elem_hex.f90:119-120).
@@ -283,5 +284,33 @@ contains
          end do
       end do
    end subroutine msod_hex_tables
+
+   ! Linear tetrahedron (TET04) with the 4-point rule of the reference driver
+   ! (main.f90:374-383): tet_4points (quadrature_rules.f90:84-99) gives the
+   ! points (a,b,b), (b,a,b), (b,b,a), (b,b,b) and weights 1/4; the P1 basis
+   ! (elem_tet.f90:9-36) N = (1-xi-eta-zeta, xi, eta, zeta) is evaluated there.
+   ! The point coordinates are the reference's 16-digit literals (not the
+   ! exact (5 +- 3 sqrt 5)/20 roots): they are an input format.
+   !   xgp(4,3), wgp(4), Ngp(4,4) (point, node), dNgp(3,4,4)
+   subroutine msod_tet_tables(xgp, wgp, Ngp, dNgp)
+      real(c_double), intent(out) :: xgp(4,3), wgp(4), Ngp(4,4), dNgp(3,4,4)
+      real(8), parameter :: pa = 0.5854101966249680d0, pb = 0.1381966011250110d0
+      integer :: g
+      xgp(1,:) = [pa, pb, pb]
+      xgp(2,:) = [pb, pa, pb]
+      xgp(3,:) = [pb, pb, pa]
+      xgp(4,:) = [pb, pb, pb]
+      wgp = 0.25d0
+      do g = 1, 4
+         Ngp(g,1) = 1d0 - xgp(g,1) - xgp(g,2) - xgp(g,3)
+         Ngp(g,2) = xgp(g,1)
+         Ngp(g,3) = xgp(g,2)
+         Ngp(g,4) = xgp(g,3)
+         dNgp(:,1,g) = [-1d0, -1d0, -1d0]
+         dNgp(:,2,g) = [1d0, 0d0, 0d0]
+         dNgp(:,3,g) = [0d0, 1d0, 0d0]
+         dNgp(:,4,g) = [0d0, 0d0, 1d0]
+      end do
+   end subroutine msod_tet_tables
 
 end module msod_fem

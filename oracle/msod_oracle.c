@@ -661,3 +661,200 @@ void msod_oracle_diffu(int p, int nelem, int npoin, const int32_t *connec, const
 #undef INV
 #undef DL
 }
+
+/* ------------------------------------------------------------------------ */
+/* Linear tetrahedra (TET04)                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* The 4-point rule of quadrature_rules.f90:84-99 (the reference's literals)
+ * and the P1 basis of elem_tet.f90:9-36 at its points, as the driver builds
+ * them (main.f90:377-383): Ngp(g,:) = N(xgp(g,:)), dNgp(:,:,g) = dN. */
+void msod_oracle_tet_tables(double *xgp, double *wgp, double *Ngp, double *dNgp)
+{
+    const double a = 0.5854101966249680, b = 0.1381966011250110;
+    const double pts[4][3] = {{a, b, b}, {b, a, b}, {b, b, a}, {b, b, b}};
+    for (int g = 0; g < 4; ++g) {
+        for (int d = 0; d < 3; ++d) xgp[g + 4 * d] = pts[g][d];
+        wgp[g] = 0.25;
+        const double xi = pts[g][0], eta = pts[g][1], zeta = pts[g][2];
+        Ngp[g + 4 * 0] = 1.0 - xi - eta - zeta;
+        Ngp[g + 4 * 1] = xi;
+        Ngp[g + 4 * 2] = eta;
+        Ngp[g + 4 * 3] = zeta;
+        for (int n = 0; n < 4; ++n)
+            for (int d = 0; d < 3; ++d)
+                dNgp[d + 3 * (n + 4 * g)] = n == 0 ? -1.0 : (d == n - 1 ? 1.0 : 0.0);
+    }
+}
+
+/* jacobian_oper.f90:138-267, ndime = 3 branch */
+void msod_oracle_tet_jacobian(int nelem, int npoin, const int32_t *connec, const double *coord,
+                              const double *dNgp, const double *wgp, double *gpvol, double *He)
+{
+    for (int64_t e = 0; e < nelem; ++e) {
+        for (int g = 0; g < 4; ++g) {
+            double Je[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double s = 0.0; /* dot_product(dNgp(i,:,g), coord(connec(e,:),j)) */
+                    for (int n = 0; n < 4; ++n)
+                        s += dNgp[i + 3 * (n + 4 * g)] * coord[(connec[e + (int64_t)nelem * n] - 1) + (int64_t)npoin * j];
+                    Je[i][j] = 0.0 + s;
+                }
+            const double det = Je[0][0] * Je[1][1] * Je[2][2] + Je[0][1] * Je[1][2] * Je[2][0] +
+                               Je[0][2] * Je[1][0] * Je[2][1] - Je[2][0] * Je[1][1] * Je[0][2] -
+                               Je[2][1] * Je[1][2] * Je[0][0] - Je[2][2] * Je[1][0] * Je[0][1];
+            double m[9];  /* a(1..9) of the reference, signs applied */
+            m[0] = Je[1][1] * Je[2][2] - Je[2][1] * Je[1][2];
+            m[1] = -(Je[1][0] * Je[2][2] - Je[2][0] * Je[1][2]);
+            m[2] = Je[1][0] * Je[2][1] - Je[2][0] * Je[1][1];
+            m[3] = -(Je[0][1] * Je[2][2] - Je[2][1] * Je[0][2]);
+            m[4] = Je[0][0] * Je[2][2] - Je[2][0] * Je[0][2];
+            m[5] = -(Je[0][0] * Je[2][1] - Je[2][0] * Je[0][1]);
+            m[6] = Je[0][1] * Je[1][2] - Je[1][1] * Je[0][2];
+            m[7] = -(Je[0][0] * Je[1][2] - Je[1][0] * Je[0][2]);
+            m[8] = Je[0][0] * Je[1][1] - Je[1][0] * Je[0][1];
+            /* b = transpose of a as 3x3 (b(2)=a(4), b(3)=a(7), ...), then /det */
+            const int tr[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};
+            double *h = He + 9 * (g + 4 * e);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) /* He(r,c) = b(3r + c) */
+                    h[r + 3 * c] = (1.0 / det) * m[tr[3 * r + c]];
+            gpvol[g + 4 * e] = wgp[g] * det;
+        }
+    }
+}
+
+/* elem_convec.f90:839-948 */
+void msod_oracle_fem_convec(int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                            const double *dNgp, const double *He, const double *gpvol,
+                            const double *u, const double *q, const double *rho, const double *pr,
+                            const double *E, double *Rmass, double *Rmom, double *Rener)
+{
+    const int64_t np = npoin, ne = nelem;
+    memset(Rmass, 0, sizeof(double) * np);
+    memset(Rmom, 0, sizeof(double) * np * 3);
+    memset(Rener, 0, sizeof(double) * np);
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t ip[4];
+        for (int n = 0; n < 4; ++n) ip[n] = connec[e + ne * n] - 1;
+        double Re_mass[4] = {0}, Re_ener[4] = {0}, Re_mom[4][3] = {{0}};
+        for (int g = 0; g < 4; ++g) {
+            double divQ = 0, divFe = 0, gradRho[3] = {0}, gradE[3] = {0}, gradP[3] = {0};
+            double gradU[3][3] = {{0}}, divF[3] = {0}, gpcar[4][3];
+            const double *h = He + 9 * (g + 4 * e);
+            for (int i = 0; i < 3; ++i)
+                for (int n = 0; n < 4; ++n) {
+                    const double *dn = dNgp + 3 * (n + 4 * g);
+                    gpcar[n][i] = h[i] * dn[0] + h[i + 3] * dn[1] + h[i + 6] * dn[2];
+                }
+            for (int i = 0; i < 3; ++i)
+                for (int n = 0; n < 4; ++n) {
+                    const int64_t k = ip[n];
+                    divQ = divQ + gpcar[n][i] * q[k + np * i];
+                    divFe = divFe + gpcar[n][i] * (E[k] + pr[k]) * u[k + np * i];
+                    gradRho[i] = gradRho[i] + gpcar[n][i] * rho[k];
+                    gradE[i] = gradE[i] + gpcar[n][i] * (E[k] + pr[k]);
+                    gradP[i] = gradP[i] + gpcar[n][i] * pr[k];
+                    for (int j = 0; j < 3; ++j) {
+                        gradU[i][j] = gradU[i][j] + gpcar[n][j] * u[k + np * i];
+                        divF[i] = divF[i] + gpcar[n][j] * q[k + np * i] * u[k + np * j];
+                    }
+                }
+            const double divU = gradU[0][0] + gradU[1][1] + gradU[2][2];
+            for (int n = 0; n < 4; ++n) {
+                const int64_t k = ip[n];
+                double aux1 = 0, aux2 = 0;
+                for (int i = 0; i < 3; ++i) {
+                    aux1 = aux1 + gradRho[i] * u[k + np * i];
+                    aux2 = aux2 + gradE[i] * u[k + np * i];
+                }
+                const double w = gpvol[g + 4 * e] * Ngp[g + 4 * n] * 0.5;
+                Re_mass[n] = Re_mass[n] + w * (divQ + rho[k] * divU + aux1);
+                Re_ener[n] = Re_ener[n] + w * (divFe + (E[k] + pr[k]) * divU + aux2);
+                for (int i = 0; i < 3; ++i) {
+                    double aux3 = 0, aux4 = 0;
+                    for (int j = 0; j < 3; ++j) {
+                        aux3 = aux3 + gradU[i][j] * q[k + np * j];
+                        aux4 = aux4 + u[k + np * i] * u[k + np * j] * gradRho[j];
+                    }
+                    Re_mom[n][i] = Re_mom[n][i] + w * (divF[i] + q[k + np * i] * divU + aux3 + aux4 + gradP[i]);
+                }
+            }
+        }
+        for (int n = 0; n < 4; ++n) {
+            Rmass[ip[n]] += Re_mass[n];
+            Rener[ip[n]] += Re_ener[n];
+            for (int i = 0; i < 3; ++i) Rmom[ip[n] + np * i] += Re_mom[n][i];
+        }
+    }
+}
+
+/* elem_diffu.f90:199-317.  As in the reference, the fluid properties at
+ * point g weight node n with Ngp(n,g) (the table is stored (point, node)). */
+void msod_oracle_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                           const double *dNgp, const double *He, const double *gpvol, double Cp,
+                           double Pr, double c_rho, double c_ener, const double *rho,
+                           const double *u, const double *Tem, const double *mu_fluid,
+                           const double *mu_e, const double *mu_sgs, double *Rmass, double *Rmom,
+                           double *Rener)
+{
+    const int64_t np = npoin, ne = nelem;
+    memset(Rmass, 0, sizeof(double) * np);
+    memset(Rmom, 0, sizeof(double) * np * 3);
+    memset(Rener, 0, sizeof(double) * np);
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t ip[4];
+        for (int n = 0; n < 4; ++n) ip[n] = connec[e + ne * n] - 1;
+        double Re_mass[4] = {0}, Re_ener[4] = {0}, Re_mom[4][3] = {{0}};
+        for (int g = 0; g < 4; ++g) {
+            double auxRho = 0, auxNu = 0, auxMu = 0, auxKappa = 0;
+            for (int n = 0; n < 4; ++n) auxRho = auxRho + Ngp[n + 4 * g] * rho[ip[n]];
+            for (int n = 0; n < 4; ++n) {
+                const double me = mu_e[e + ne * n], ms = mu_sgs[e + ne * n], mf = mu_fluid[ip[n]];
+                auxNu = auxNu + Ngp[n + 4 * g] * me * (c_rho / auxRho);
+                auxMu = auxMu + Ngp[n + 4 * g] * (mf + auxRho * ms + me);
+                auxKappa = auxKappa + Ngp[n + 4 * g] * (mf * (Cp / Pr) + c_ener * me / 0.4 + auxRho * ms / 0.9);
+            }
+            double gpcar[4][3];
+            const double *h = He + 9 * (g + 4 * e);
+            for (int i = 0; i < 3; ++i)
+                for (int n = 0; n < 4; ++n) {
+                    const double *dn = dNgp + 3 * (n + 4 * g);
+                    gpcar[n][i] = h[i] * dn[0] + h[i + 3] * dn[1] + h[i + 6] * dn[2];
+                }
+            double gradRho[3] = {0}, gradT[3] = {0}, gradU[3][3] = {{0}};
+            for (int i = 0; i < 3; ++i)
+                for (int n = 0; n < 4; ++n) {
+                    const int64_t k = ip[n];
+                    gradRho[i] = gradRho[i] + gpcar[n][i] * rho[k];
+                    gradT[i] = gradT[i] + gpcar[n][i] * Tem[k];
+                    for (int j = 0; j < 3; ++j) gradU[i][j] = gradU[i][j] + gpcar[n][i] * u[k + np * j];
+                }
+            const double divU = gradU[0][0] + gradU[1][1] + gradU[2][2];
+            double tau[3][3] = {{0}}, tauU[3] = {0};
+            for (int i = 0; i < 3; ++i) {
+                tau[i][i] = -(2.0 / 3.0) * divU;
+                for (int j = 0; j < 3; ++j) tau[i][j] = tau[i][j] + gradU[i][j] + gradU[j][i];
+            }
+            for (int j = 0; j < 3; ++j) {
+                double auxU = 0;
+                for (int n = 0; n < 4; ++n) auxU = auxU + Ngp[n + 4 * g] * u[ip[n] + np * j];
+                for (int i = 0; i < 3; ++i) tauU[i] = tauU[i] + tau[i][j] * auxU;
+            }
+            const double gv = gpvol[g + 4 * e];
+            for (int i = 0; i < 3; ++i)
+                for (int n = 0; n < 4; ++n) {
+                    Re_mass[n] = Re_mass[n] + gv * auxNu * gpcar[n][i] * gradRho[i];
+                    Re_ener[n] = Re_ener[n] + gv * gpcar[n][i] * (auxMu * tauU[i] + auxKappa * gradT[i]);
+                    for (int j = 0; j < 3; ++j)
+                        Re_mom[n][i] = Re_mom[n][i] + gv * gpcar[n][j] * auxMu * tau[i][j];
+                }
+        }
+        for (int n = 0; n < 4; ++n) {
+            Rmass[ip[n]] += Re_mass[n];
+            Rener[ip[n]] += Re_ener[n];
+            for (int i = 0; i < 3; ++i) Rmom[ip[n] + np * i] += Re_mom[n][i];
+        }
+    }
+}

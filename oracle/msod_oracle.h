@@ -61,6 +61,27 @@ void msod_oracle_diffu(int p, int nelem, int npoin, const int32_t *connec, const
                        const double *mu_sgs, double *Rmass, double *Rmom, double *Rener,
                        int nthreads);
 
+/* Linear tetrahedra (TET04).  Ngp(4,4) (point, node) Ngp[g + 4*n];
+ * dNgp(3,4,4) dNgp[d + 3*(n + 4*g)]; He(3,3,4,nelem); gpvol(1,4,nelem);
+ * connec(nelem,4); mu_e, mu_sgs(nelem,4).  Outputs are overwritten. */
+/* tet_4points (quadrature_rules.f90:84-99) and tet_04 (elem_tet.f90:9-36) */
+void msod_oracle_tet_tables(double *xgp, double *wgp, double *Ngp, double *dNgp);
+/* tet_jacobian (jacobian_oper.f90:138-267) */
+void msod_oracle_tet_jacobian(int nelem, int npoin, const int32_t *connec, const double *coord,
+                              const double *dNgp, const double *wgp, double *gpvol, double *He);
+/* fem_convec (elem_convec.f90:839-948) */
+void msod_oracle_fem_convec(int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                            const double *dNgp, const double *He, const double *gpvol,
+                            const double *u, const double *q, const double *rho, const double *pr,
+                            const double *E, double *Rmass, double *Rmom, double *Rener);
+/* fem_diffu (elem_diffu.f90:199-317) */
+void msod_oracle_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ngp,
+                           const double *dNgp, const double *He, const double *gpvol, double Cp,
+                           double Pr, double c_rho, double c_ener, const double *rho,
+                           const double *u, const double *Tem, const double *mu_fluid,
+                           const double *mu_e, const double *mu_sgs, double *Rmass, double *Rmom,
+                           double *Rener);
+
 /* Learned closure (msod_oracle_mlp.c): the MLP mu_sgs = scale*softplus(z3)
  * of a 1530-float parameter block, per sample of 12 features; quant = 1
  * rounds to bfloat16 where the GPU feeds its matrix cores. */

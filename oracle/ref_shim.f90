@@ -15,6 +15,11 @@
 !   sodref_full_convec_ijk           elem_convec.f90:198-431
 !   sodref_full_convec_ijk_split     elem_convec.f90:433-836
 !   sodref_full_diffusion_ijk        elem_diffu.f90:10-197
+!   sodref_tet_tables                main.f90:374-383 (tet_4points, tet_04)
+!   sodref_gen_tet_mesh              mod_mesh.f90:71-125
+!   sodref_tet_jacobian              jacobian_oper.f90:138-267
+!   sodref_fem_convec                elem_convec.f90:839-948
+!   sodref_fem_diffu                 elem_diffu.f90:199-317
 module sodref_shim
 
    use iso_c_binding
@@ -23,11 +28,12 @@ module sodref_shim
    use elem_qua, only: init_basic_qua, quad_order_edges
    use elem_hex, only: init_basic_hex, hex_highorder, hex_order_edges
    use mod_gmsh_indices, only: genHighOrderHex
-   use quadrature_rules, only: GaussLobattoLegendre_hex
-   use mod_mesh, only: create_mesh
-   use jacobian_oper, only: elem_jacobian
-   use elem_convec, only: full_convec_ijk, full_convec_ijk_split
-   use elem_diffu, only: full_diffusion_ijk
+   use elem_tet, only: tet_04
+   use quadrature_rules, only: GaussLobattoLegendre_hex, tet_4points
+   use mod_mesh, only: create_mesh, gen_tet_mesh
+   use jacobian_oper, only: elem_jacobian, tet_jacobian
+   use elem_convec, only: full_convec_ijk, full_convec_ijk_split, fem_convec
+   use elem_diffu, only: full_diffusion_ijk, fem_diffu
 
    implicit none
 
@@ -191,5 +197,65 @@ contains
                               AtoI, AtoJ, AtoK, Cp_r, Pr_r, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
                               Rmass, Rmom, Rener)
    end subroutine sodref_full_diffusion_ijk
+
+   ! TET04 tables as the driver builds them (main.f90:374-383).
+   subroutine sodref_tet_tables(xgp, wgp, Ngp, dNgp) bind(C, name='sodref_tet_tables')
+      real(c_double), intent(out) :: xgp(4,ndime), wgp(4), Ngp(4,4), dNgp(ndime,4,4)
+      real(rp) :: x(4,ndime), w(4), N(4,4), dN(ndime,4,4)
+      integer(4) :: igaus
+      call tet_4points(x, w)
+      do igaus = 1,4
+         call tet_04(x(igaus,1), x(igaus,2), x(igaus,3), N(igaus,:), dN(:,:,igaus))
+      end do
+      xgp = real(x, c_double)
+      wgp = real(w, c_double)
+      Ngp = real(N, c_double)
+      dNgp = real(dN, c_double)
+   end subroutine sodref_tet_tables
+
+   subroutine sodref_gen_tet_mesh(nelem, npoin, xyzBase, connec, xyz) bind(C, name='sodref_gen_tet_mesh')
+      integer(c_int), value :: nelem, npoin
+      real(c_double), intent(in) :: xyzBase(4,ndime)
+      integer(c_int), intent(out) :: connec(nelem,4)
+      real(c_double), intent(out) :: xyz(npoin,ndime)
+      call gen_tet_mesh(nelem, npoin, xyzBase, connec, xyz)
+   end subroutine sodref_gen_tet_mesh
+
+   subroutine sodref_tet_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He) &
+         bind(C, name='sodref_tet_jacobian')
+      integer(c_int), value :: nelem, npoin
+      integer(c_int), intent(in) :: connec(nelem,4)
+      real(c_double), intent(in) :: coord(npoin,ndime), dNgp(ndime,4,4), wgp(4)
+      real(c_double), intent(out) :: gpvol(1,4,nelem), He(ndime,ndime,4,nelem)
+      call tet_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He)
+   end subroutine sodref_tet_jacobian
+
+   subroutine sodref_fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, &
+         Rmass, Rmom, Rener) bind(C, name='sodref_fem_convec')
+      integer(c_int), value :: nelem, npoin
+      integer(c_int), intent(in) :: connec(nelem,4)
+      real(c_double), intent(in) :: Ngp(4,4), dNgp(ndime,4,4), He(ndime,ndime,4,nelem), gpvol(1,4,nelem)
+      real(c_double), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
+      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      call fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, Rmass, Rmom, Rener)
+   end subroutine sodref_fem_convec
+
+   subroutine sodref_fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp, Pr, c_rho_in, c_ener_in, &
+         rho, u, Tem, mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener) bind(C, name='sodref_fem_diffu')
+      integer(c_int), value :: nelem, npoin
+      integer(c_int), intent(in) :: connec(nelem,4)
+      real(c_double), intent(in) :: Ngp(4,4), dNgp(ndime,4,4), He(ndime,ndime,4,nelem), gpvol(1,4,nelem)
+      real(c_double), value :: Cp, Pr, c_rho_in, c_ener_in
+      real(c_double), intent(in) :: rho(npoin), u(npoin,ndime), Tem(npoin), mu_fluid(npoin)
+      real(c_double), intent(in) :: mu_e(nelem,4), mu_sgs(nelem,4)
+      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp) :: Cp_r, Pr_r
+      c_rho = c_rho_in
+      c_ener = c_ener_in
+      Cp_r = Cp
+      Pr_r = Pr
+      call fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp_r, Pr_r, rho, u, Tem, mu_fluid, &
+                     mu_e, mu_sgs, Rmass, Rmom, Rener)
+   end subroutine sodref_fem_diffu
 
 end module sodref_shim

@@ -15,6 +15,12 @@ tell a generator change from a kernel change.  Fixtures:
   jac_p4.npz      He, gpvol of elem_jacobian on a jittered 2x2x2 p=4 box
   case_*.npz      Rmass/Rmom of full_convec_ijk, Rener of full_convec_ijk_split,
                   Dmass/Dmom/Dener of full_diffusion_ijk
+  tet_tables.npz  TET04 tables of the reference driver (main.f90:374-383)
+  tet_mesh.npz    digests of gen_tet_mesh's connec and coord (mod_mesh.f90:71-125)
+  tet_jac.npz     He, gpvol of tet_jacobian on a jittered tetrahedral mesh
+  tet_case_*.npz  Rmass/Rmom/Rener of fem_convec, Dmass/Dmom/Dener of fem_diffu
+
+`python tests/golden/make_golden.py --tet` writes the TET fixtures only.
 """
 import hashlib
 import json
@@ -29,7 +35,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
 import minisod2d_amd as m  # noqa: E402
-from oracle_lib import RefLib  # noqa: E402
+from oracle_lib import RefLib, RefTet  # noqa: E402
 
 # (name, builder) -- small enough for the reference to finish in seconds
 CASES = {
@@ -45,6 +51,18 @@ CASES = {
     "case_refdriver_p3": lambda: m.reference_problem(5, 3),
 }
 
+# (name, builder, c_rho, c_ener) -- the TET04 kernels on the reference driver's
+# disjoint tetrahedral mesh over the nodal fields of a hex problem
+TET_CASES = {
+    "tet_case_refdriver_p3": (lambda: m.tet_problem(m.reference_problem(4, 3)), 1.0, 1.0),
+    "tet_case_box3_p2_jit": (lambda: m.tet_problem(m.box_problem(3, 2, jitter=0.05, rho_jump=0.1),
+                                                   jitter=0.3, random_mu=True, strict=False), 1.0, 1.0),
+    "tet_case_box4_p3_coef": (lambda: m.tet_problem(m.box_problem(4, 3), jitter=0.2, random_mu=True,
+                                                    strict=False), 0.7, 1.3),
+}
+TET_INPUT_FIELDS = ("connec", "coord", "He", "gpvol", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid",
+                    "mu_e", "mu_sgs")
+
 INPUT_FIELDS = ("connec", "He", "gpvol", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")
 
 
@@ -56,7 +74,38 @@ def input_digests(prob) -> dict:
     return {k: digest(getattr(prob, k)) for k in INPUT_FIELDS}
 
 
+def tet_input_digests(tp) -> dict:
+    return {k: digest(getattr(tp, k)) for k in TET_INPUT_FIELDS}
+
+
+def write_tet():
+    ref = RefTet(4)
+    t = ref.tables()
+    np.savez_compressed(os.path.join(HERE, "tet_tables.npz"), **t)
+    nelem = 50
+    connec, xyz = ref.mesh(nelem, 4 * nelem + 3, m.TET_BASE)
+    np.savez_compressed(os.path.join(HERE, "tet_mesh.npz"), nelem=nelem, npoin=4 * nelem + 3,
+                        connec_sha=digest(connec), coord_sha=digest(xyz[:4 * nelem]))
+    tp = m.tet_problem(m.box_problem(3, 2), jitter=0.4, strict=False)
+    He, gp = ref.jacobian(tp.nelem, tp.npoin, tp.connec, tp.coord, t["dNgp"], t["wgp"])
+    np.savez_compressed(os.path.join(HERE, "tet_jac.npz"), He=He, gpvol=gp, coord_sha=digest(tp.coord),
+                        connec_sha=digest(tp.connec))
+    for name, (build, c_rho, c_ener) in TET_CASES.items():
+        tp = build()
+        Rmass, Rmom, Rener = ref.fem_convec(tp)
+        Dmass, Dmom, Dener = ref.fem_diffu(tp, c_rho, c_ener)
+        meta = dict(tp.meta, nelem=tp.nelem, npoin=tp.npoin, Cp=tp.Cp, Pr=tp.Pr, c_rho=c_rho,
+                    c_ener=c_ener, inputs=tet_input_digests(tp))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), Rmass=Rmass, Rmom=Rmom, Rener=Rener,
+                            Dmass=Dmass, Dmom=Dmom, Dener=Dener, meta=json.dumps(meta))
+        print(name, tp.nelem, tp.npoin, "ok")
+
+
 def main():
+    if "--tet" in sys.argv[1:]:
+        write_tet()
+        return
+    write_tet()
     for p in range(2, 9):
         t = RefLib(p).tables()
         np.savez_compressed(os.path.join(HERE, f"tables_p{p}.npz"),

@@ -36,6 +36,10 @@ class Oracle:
         L.msod_oracle_mlp_forward.argtypes = [_vp, _vp, ctypes.c_int64, ctypes.c_int, _vp, _vp]
         L.msod_oracle_mlp_gradient.argtypes = [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, _vp]
         L.msod_oracle_adam.argtypes = [_vp] * 4 + [ctypes.c_int, _d, _d, _d, _d]
+        L.msod_oracle_tet_tables.argtypes = [_vp] * 4
+        L.msod_oracle_tet_jacobian.argtypes = [ctypes.c_int] * 2 + [_vp] * 6
+        L.msod_oracle_fem_convec.argtypes = [ctypes.c_int] * 2 + [_vp] * 13
+        L.msod_oracle_fem_diffu.argtypes = [ctypes.c_int] * 2 + [_vp] * 5 + [_d] * 4 + [_vp] * 9
         self.L = L
 
     def gll(self, p, exact=False):
@@ -77,6 +81,32 @@ class Oracle:
                 prob.mu_sgs if mu_sgs is None else mu_sgs)), *map(_p, R), nthreads)
         return R
 
+
+    # ---- linear tetrahedra ----------------------------------------------------
+    def tet_tables(self):
+        t = [_f((4, 3)), _f(4), _f((4, 4)), _f((3, 4, 4))]
+        self.L.msod_oracle_tet_tables(*[_vp(x.ctypes.data) for x in t])
+        return dict(zip("xgp wgp Ngp dNgp".split(), t))
+
+    def tet_jacobian(self, nelem, npoin, connec, coord, dNgp, wgp):
+        He, gp = _f((3, 3, 4, nelem)), _f((1, 4, nelem))
+        self.L.msod_oracle_tet_jacobian(nelem, npoin, *map(_p, (connec, coord, dNgp, wgp, gp, He)))
+        return He, gp
+
+    def fem_convec(self, tp):
+        R = tp.outputs()
+        self.L.msod_oracle_fem_convec(tp.nelem, tp.npoin, *map(_p, (
+            tp.connec, tp.tables.Ngp, tp.tables.dNgp, tp.He, tp.gpvol, tp.u, tp.q, tp.rho, tp.pr,
+            tp.E)), *map(_p, R))
+        return R
+
+    def fem_diffu(self, tp, c_rho=1.0, c_ener=1.0, Ngp=None):
+        R = tp.outputs()
+        self.L.msod_oracle_fem_diffu(tp.nelem, tp.npoin, *map(_p, (
+            tp.connec, tp.tables.Ngp if Ngp is None else Ngp, tp.tables.dNgp, tp.He, tp.gpvol)),
+            tp.Cp, tp.Pr, c_rho, c_ener, *map(_p, (
+                tp.rho, tp.u, tp.Tem, tp.mu_fluid, tp.mu_e, tp.mu_sgs)), *map(_p, R))
+        return R
 
     # ---- learned closure (oracle/msod_oracle_mlp.c) ---------------------------
     def mlp_forward(self, P, feat, quant=1):
@@ -123,6 +153,11 @@ class RefLib:
         L.sodref_full_diffusion_ijk.argtypes = [ctypes.c_int] * 2 + [_vp] * 11 + [_d] * 4 + [_vp] * 9
         L.sodref_elem_jacobian.argtypes = [ctypes.c_int] * 2 + [_vp] * 6
         L.sodref_create_mesh.argtypes = [ctypes.c_int] * 2 + [_vp] * 3
+        L.sodref_tet_tables.argtypes = [_vp] * 4
+        L.sodref_gen_tet_mesh.argtypes = [ctypes.c_int] * 2 + [_vp] * 3
+        L.sodref_tet_jacobian.argtypes = [ctypes.c_int] * 2 + [_vp] * 6
+        L.sodref_fem_convec.argtypes = [ctypes.c_int] * 2 + [_vp] * 13
+        L.sodref_fem_diffu.argtypes = [ctypes.c_int] * 2 + [_vp] * 5 + [_d] * 4 + [_vp] * 9
         self.L = L
         self.p = p
 
@@ -164,6 +199,43 @@ class RefLib:
                                               t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK)),
             prob.Cp, prob.Pr, c_rho, c_ener,
             *map(_p, (prob.rho, prob.u, prob.Tem, prob.mu_fluid, prob.mu_e, prob.mu_sgs)), *map(_p, R))
+        return R
+
+
+class RefTet:
+    """The reference's TET04 routines (any porder's library holds them)."""
+
+    def __init__(self, p=4):
+        self.R = RefLib(p)
+        self.L = self.R.L
+
+    def tables(self):
+        t = [_f((4, 3)), _f(4), _f((4, 4)), _f((3, 4, 4))]
+        self.L.sodref_tet_tables(*[_vp(x.ctypes.data) for x in t])
+        return dict(zip("xgp wgp Ngp dNgp".split(), t))
+
+    def mesh(self, nelem, npoin, base):
+        connec, xyz = _f((nelem, 4), np.int32), _f((npoin, 3))
+        self.L.sodref_gen_tet_mesh(nelem, npoin, _p(np.asfortranarray(base)), _p(connec), _p(xyz))
+        return connec, xyz
+
+    def jacobian(self, nelem, npoin, connec, coord, dNgp, wgp):
+        He, gp = _f((3, 3, 4, nelem)), _f((1, 4, nelem))
+        self.L.sodref_tet_jacobian(nelem, npoin, *map(_p, (connec, coord, dNgp, wgp, gp, He)))
+        return He, gp
+
+    def fem_convec(self, tp):
+        R = tp.outputs()
+        self.L.sodref_fem_convec(tp.nelem, tp.npoin, *map(_p, (
+            tp.connec, tp.tables.Ngp, tp.tables.dNgp, tp.He, tp.gpvol, tp.u, tp.q, tp.rho, tp.pr,
+            tp.E)), *map(_p, R))
+        return R
+
+    def fem_diffu(self, tp, c_rho=1.0, c_ener=1.0):
+        R = tp.outputs()
+        self.L.sodref_fem_diffu(tp.nelem, tp.npoin, *map(_p, (
+            tp.connec, tp.tables.Ngp, tp.tables.dNgp, tp.He, tp.gpvol)), tp.Cp, tp.Pr, c_rho, c_ener,
+            *map(_p, (tp.rho, tp.u, tp.Tem, tp.mu_fluid, tp.mu_e, tp.mu_sgs)), *map(_p, R))
         return R
 
 

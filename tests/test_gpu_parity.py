@@ -6,6 +6,7 @@ against the oracle on the same rounded inputs; against the oracle on the FP64
 inputs, <= 1e-6 for FP32 element arrays (geo32) and the FP32 rounding of the
 inputs (<= 4e-6 at these sizes) for FP32 storage of everything (mixed)."""
 import os
+import re
 import sys
 
 import numpy as np
@@ -342,6 +343,7 @@ def test_shuffled_elements_and_unreferenced_nodes(gpu):
     (["--nelem", "6", "--porder", "4"], "case_refdriver_p4"),
     (["--n", "8", "--porder", "3"], "case_box8_p3"),
     (["--n", "8", "--porder", "3", "--gpu-geometry"], "case_box8_p3"),
+    (["--nelem", "5", "--porder", "3"], "case_refdriver_p3"),
 ])
 def test_fortran_driver(gpu, tmp_path, args, case):
     """The Fortran timer driver (drop-in elem_convec/elem_diffu modules over the
@@ -349,7 +351,7 @@ def test_fortran_driver(gpu, tmp_path, args, case):
     import subprocess
     exe = os.path.join(m.LIB_DIR, "miniSOD")
     say(f"[gpu] Fortran driver {' '.join(args)}")
-    r = subprocess.run([exe, *args, "--nruns", "3", "--quiet"], cwd=tmp_path, capture_output=True,
+    r = subprocess.run([exe, *args, "--nruns", "3"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     gold, _ = load_case(case)
@@ -361,6 +363,45 @@ def test_fortran_driver(gpu, tmp_path, args, case):
     assert rel_l2(rd[:, 1], gold["Dmass"]) <= TOL64
     assert rel_l2(rd[:, 2:5], gold["Dmom"]) <= TOL64
     assert rel_l2(rd[:, 5], gold["Dener"]) <= TOL64
+    # timers.dat records: format (i3,1x,4(f16.8,1x)) of main.f90:495,499
+    lines = (tmp_path / "timers.dat").read_text().splitlines()
+    assert len(lines) == 3
+    for i, line in enumerate(lines, 1):
+        assert re.fullmatch(r"[ \d]{2}\d( [ \d-]{6}\d\.\d{8}){4} ?", line), line
+        assert int(line[:3]) == i
     timers = np.loadtxt(tmp_path / "timers.dat")
     assert timers.shape == (3, 5) and np.all(timers[:, 1] > 0) and np.all(timers[:, 3] > 0)
     assert "GLL-node updates/s" in r.stdout
+    # the summary block of main.f90:544-566 quotes the extremes of the dumps
+    summary = r.stdout[r.stdout.index("Basic results:"):].split("Basic TET results:")[0]
+    cols = {"Rmass": (rc, 1), "Rmom(:,1)": (rc, 2), "Rmom(:,2)": (rc, 3), "Rmom(:,3)": (rc, 4),
+            "Rener": (rc, 5), "Dmass": (rd, 1), "Dmom(:,1)": (rd, 2), "Dmom(:,2)": (rd, 3),
+            "Dmom(:,3)": (rd, 4), "Dener": (rd, 5)}
+    num = r"([-+0-9.Ee]+)"
+    for name, (arr, j) in cols.items():
+        hit = re.search(rf"Max {re.escape(name)}\s*=\s*{num}\s*Min {re.escape(name)}\s*=\s*{num}", summary)
+        assert hit, name
+        assert float(hit.group(1)) == pytest.approx(arr[:, j].max(), rel=1e-14, abs=1e-300)
+        assert float(hit.group(2)) == pytest.approx(arr[:, j].min(), rel=1e-14, abs=1e-300)
+    # TET04 part (main.f90:364-407, 465-491, 567-581): runs when 4 divides npoin
+    from golden.make_golden import CASES as BUILDERS
+    prob = BUILDERS[case]()
+    if prob.npoin % 4:
+        assert "Basic TET results:" not in r.stdout and np.all(timers[:, [2, 4]] == 0)
+        return
+    assert np.all(timers[:, 2] > 0) and np.all(timers[:, 4] > 0)
+    tp = m.tet_problem(prob)
+    o = Oracle()
+    want = dict(zip(KEYS, o.fem_convec(tp) + o.fem_diffu(tp)))
+    tsum = r.stdout[r.stdout.index("Basic TET results:"):]
+    cols = {"Rmass": ("Rmass", None), "Rmom(:,1)": ("Rmom", 0), "Rmom(:,2)": ("Rmom", 1),
+            "Rmom(:,3)": ("Rmom", 2), "Rener": ("Rener", None), "Dmass": ("Dmass", None),
+            "Dmom(:,1)": ("Dmom", 0), "Dmom(:,2)": ("Dmom", 1), "Dmom(:,3)": ("Dmom", 2),
+            "Dener": ("Dener", None)}
+    for name, (key, j) in cols.items():
+        arr = want[key] if j is None else want[key][:, j]
+        hit = re.search(rf"Max {re.escape(name)}\s*=\s*{num}\s*Min {re.escape(name)}\s*=\s*{num}", tsum)
+        assert hit, name
+        scale = np.abs(arr).max()
+        assert float(hit.group(1)) == pytest.approx(arr.max(), rel=1e-12, abs=1e-12 * scale), name
+        assert float(hit.group(2)) == pytest.approx(arr.min(), rel=1e-12, abs=1e-12 * scale), name
