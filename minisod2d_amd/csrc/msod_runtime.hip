@@ -105,6 +105,16 @@ hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64
     return hipSuccess;
 }
 
+// Zero n doubles at a 16-byte aligned p (a kernel, not a memset: the calls
+// stay safe to capture in a HIP graph).
+hipError_t launch_zero_doubles(double *p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(n / 2 / 256 + 1, 0x7fffffff);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
                              const uint32_t *nodes, int64_t count, hipStream_t s)
 {
@@ -200,11 +210,22 @@ struct DerivedGeo {
     void *geo, *gvol;
 };
 
-// A tetrahedral connectivity found valid (every entry in 1..npoin).
+// A tetrahedral connectivity found valid (every entry in 1..npoin), with
+// how its element residuals reach the nodes: plain stores when no node is
+// shared (then the nodes no element references are zeroed from a list),
+// atomics into zero-filled outputs otherwise.
 struct CheckedMesh {
     const void *src;
     uint64_t ver;
     int64_t n, npoin;
+    bool disjoint;
+    uint32_t *unref;  // device list of unreferenced nodes (disjoint meshes)
+    int64_t nunref;
+    void release()
+    {
+        if (unref) (void)hipFree(unref);
+        unref = nullptr;
+    }
 };
 
 constexpr int kNumTimers = 14;
@@ -515,7 +536,10 @@ void drop_derived_in(uintptr_t lo, uintptr_t hi)
             g.plans.erase(g.plans.begin() + i);
         }
     for (size_t i = g.checked.size(); i-- > 0;)
-        if (inside(g.checked[i].src)) g.checked.erase(g.checked.begin() + i);
+        if (inside(g.checked[i].src)) {
+            g.checked[i].release();
+            g.checked.erase(g.checked.begin() + i);
+        }
     for (size_t i = g.geos.size(); i-- > 0;)
         if (inside(g.geos[i].srcHe) || inside(g.geos[i].srcVol)) {
             (void)hipFree(g.geos[i].geo);
@@ -1385,27 +1409,66 @@ static int tet_tables(const double *Ngp, const double *dNgp, const double *wgp, 
     return MSOD_OK;
 }
 
-// connec(nelem,4) entries in 1..npoin, checked on the device once per array
-// version (the kernels clamp in any case, so they never read out of bounds)
-static int check_tet_connec(const void *dc, uint64_t ver, int64_t nelem, int64_t npoin)
+// connec(nelem,4) analysed on the device once per array version: entries
+// outside 1..npoin are an error (the kernels clamp in any case, so they never
+// read out of bounds); a mesh whose nodes each belong to at most one element
+// node (the reference's TET04 mesh) gets plain stores and the list of the
+// nodes it leaves out.
+static int check_tet_connec(const void *dc, uint64_t ver, int64_t nelem, int64_t npoin,
+                            const CheckedMesh **out)
 {
     const int64_t n = 4 * nelem;
     for (const CheckedMesh &c : g.checked)
-        if (c.src == dc && c.ver == ver && c.n == n && c.npoin == npoin) return MSOD_OK;
-    std::vector<unsigned long long> count(1, 0ull);
-    DevTemp dk;
+        if (c.src == dc && c.ver == ver && c.n == n && c.npoin == npoin) {
+            *out = &c;
+            return MSOD_OK;
+        }
+    std::vector<unsigned long long> stats(4, 0ull);
+    DevTemp dk, drefs;
     int st;
-    if ((st = dev_temp(count, dk, "connectivity check"))) return st;
-    HIPCHK(launch_connec_check((const int32_t *)dc, n, npoin, (unsigned long long *)dk.p, g.stream),
-           "connectivity check launch");
-    HIPCHK(hipMemcpyAsync(count.data(), dk.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, g.stream),
-           "connectivity check");
-    HIPCHK(hipStreamSynchronize(g.stream), "connectivity check");
-    if (count[0])
-        return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%lld)", count[0],
+    if ((st = dev_temp(stats, dk, "mesh analysis"))) return st;
+    unsigned long long *ds = (unsigned long long *)dk.p;
+    const int64_t nwords = (npoin + 1) / 2;  // the counters, zeroed as doubles
+    HIPCHK(hipMalloc(&drefs.p, sizeof(double) * nwords), "mesh analysis buffer");
+    HIPCHK(launch_zero_doubles((double *)drefs.p, nwords, g.stream), "mesh analysis");
+    HIPCHK(launch_count_refs((const int32_t *)dc, n, npoin, (uint32_t *)drefs.p, ds, g.stream),
+           "mesh analysis launch");
+    HIPCHK(launch_ref_summary((const uint32_t *)drefs.p, npoin, ds, g.stream), "mesh analysis launch");
+    HIPCHK(hipMemcpyAsync(stats.data(), ds, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, g.stream),
+           "mesh analysis");
+    HIPCHK(hipStreamSynchronize(g.stream), "mesh analysis");
+    if (stats[0])
+        return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%lld)", stats[0],
                     (long long)npoin);
-    if (g.checked.size() >= kMaxDerived) g.checked.erase(g.checked.begin());
-    g.checked.push_back({dc, ver, n, npoin});
+    CheckedMesh c{dc, ver, n, npoin, stats[1] == 0, nullptr, 0};
+    if (c.disjoint && stats[2] > 0) {
+        c.nunref = (int64_t)stats[2];
+        HIPCHK(hipMalloc((void **)&c.unref, sizeof(uint32_t) * c.nunref), "unreferenced-node list");
+        hipError_t e = launch_list_unreferenced((const uint32_t *)drefs.p, npoin, c.unref, ds + 3, g.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+        if (e != hipSuccess) {
+            c.release();
+            return hipfail(e, "unreferenced-node list");
+        }
+    }
+    if (g.checked.size() >= kMaxDerived) {
+        g.checked.front().release();
+        g.checked.erase(g.checked.begin());
+    }
+    g.checked.push_back(c);
+    *out = &g.checked.back();
+    return MSOD_OK;
+}
+
+// Outputs before a TET04 kernel: zeroed where the kernel will not store.
+static int tet_zero(const CheckedMesh &cm, double *Rmass, double *Rmom, double *Rener, int64_t npoin)
+{
+    TimedLaunch tz(0);
+    if (!cm.disjoint)
+        HIPCHK(launch_zero_outputs(Rmass, Rmom, Rener, npoin, g.stream), "zero-fill launch");
+    else if (cm.nunref > 0)
+        HIPCHK(launch_zero_nodes(Rmass, Rmom, Rener, npoin, cm.unref, cm.nunref, g.stream), "zero-fill launch");
+    tz.done();
     return MSOD_OK;
 }
 
@@ -1490,7 +1553,8 @@ int msod_fem_convec(int nelem, int npoin, const int32_t *connec, const double *N
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * NP * 3, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
-    if ((st = check_tet_connec(dc, vc, NE, NP))) return st;
+    const CheckedMesh *cm;
+    if ((st = check_tet_connec(dc, vc, NE, NP, &cm))) return st;
     a.nelem = NE;
     a.npoin = NP;
     a.connec = (const int32_t *)dc;
@@ -1504,11 +1568,8 @@ int msod_fem_convec(int nelem, int npoin, const int32_t *connec, const double *N
     a.Rmass = (double *)dRm;
     a.Rmom = (double *)dRmo;
     a.Rener = (double *)dRe;
-    {
-        TimedLaunch tz(0);
-        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
-        tz.done();
-    }
+    a.store = cm->disjoint;
+    if ((st = tet_zero(*cm, a.Rmass, a.Rmom, a.Rener, NP))) return st;
     TimedLaunch tk(11);
     HIPCHK(launch_fem_convec(a, g.stream), "TET convection kernel launch");
     tk.done();
@@ -1544,7 +1605,8 @@ int msod_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ng
     if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
     if ((st = resolve(Rmom, sizeof(double) * NP * 3, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
-    if ((st = check_tet_connec(dc, vc, NE, NP))) return st;
+    const CheckedMesh *cm;
+    if ((st = check_tet_connec(dc, vc, NE, NP, &cm))) return st;
     a.nelem = NE;
     a.npoin = NP;
     a.connec = (const int32_t *)dc;
@@ -1563,11 +1625,8 @@ int msod_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ng
     a.Rmass = (double *)dRm;
     a.Rmom = (double *)dRmo;
     a.Rener = (double *)dRe;
-    {
-        TimedLaunch tz(0);
-        HIPCHK(launch_zero_outputs(a.Rmass, a.Rmom, a.Rener, NP, g.stream), "zero-fill launch");
-        tz.done();
-    }
+    a.store = cm->disjoint;
+    if ((st = tet_zero(*cm, a.Rmass, a.Rmom, a.Rener, NP))) return st;
     TimedLaunch tk(12);
     HIPCHK(launch_fem_diffu(a, g.stream), "TET diffusion kernel launch");
     tk.done();

@@ -26,6 +26,7 @@ struct TetConvecArgs {
     const double *He, *gpvol;
     const double *u, *q, *rho, *pr, *E;
     double *Rmass, *Rmom, *Rener;
+    int store;  // 1: every node has at most one element node (plain stores)
     TetTables t;
 };
 
@@ -36,6 +37,7 @@ struct TetDiffuArgs {
     const double *rho, *u, *Tem, *mu_fluid, *mu_e, *mu_sgs;
     double Cp, Pr, c_rho, c_ener;
     double *Rmass, *Rmom, *Rener;
+    int store;
     TetTables t;
 };
 
@@ -48,13 +50,26 @@ struct TetJacobianArgs {
     TetTables t;
 };
 
-// Kernels add into zero-filled outputs with FP64 atomics (any mesh; on the
-// reference's disjoint mesh every node receives exactly one contribution).
+// Element residuals reach the nodes in one of two ways (store):
+//   0: FP64 atomics into zero-filled outputs (any mesh);
+//   1: plain stores, for a mesh where no node belongs to two element nodes
+//      (the reference's disjoint TET04 mesh): the value stored is the one an
+//      atomic add onto zero would leave.  Nodes no element references must
+//      be zeroed by the caller.
 hipError_t launch_fem_convec(const TetConvecArgs &a, hipStream_t s);
 hipError_t launch_fem_diffu(const TetDiffuArgs &a, hipStream_t s);
 hipError_t launch_tet_jacobian(const TetJacobianArgs &a, hipStream_t s);
-// *count += number of connec entries (n of them) outside 1..npoin.
-hipError_t launch_connec_check(const int32_t *connec, int64_t n, int64_t npoin,
-                               unsigned long long *count, hipStream_t s);
+
+// Mesh analysis, once per connectivity: refs[npoin] (zeroed by the caller)
+// counts the element nodes of every node; stats[0] += entries outside
+// 1..npoin (not counted), stats[1] += nodes with refs > 1, stats[2] += nodes
+// with refs == 0; list_unreferenced writes those nodes (0-based, any order)
+// to list[0 .. stats[2]) using *cursor (zeroed by the caller).
+hipError_t launch_count_refs(const int32_t *connec, int64_t n, int64_t npoin, uint32_t *refs,
+                             unsigned long long *stats, hipStream_t s);
+hipError_t launch_ref_summary(const uint32_t *refs, int64_t npoin, unsigned long long *stats,
+                              hipStream_t s);
+hipError_t launch_list_unreferenced(const uint32_t *refs, int64_t npoin, uint32_t *list,
+                                    unsigned long long *cursor, hipStream_t s);
 
 }  // namespace msod

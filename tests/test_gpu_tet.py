@@ -92,6 +92,32 @@ def test_matches_oracle_ragged(gpu, nelem):
         assert np.all(np.asarray(got)[4 * nelem:] == 0.0), key
 
 
+def test_shared_node_mesh(gpu):
+    """Tetrahedra sharing nodes (random connectivity): the element residuals
+    meet at the nodes through atomics, and nodes no element uses stay 0."""
+    say("[gpu] TET mesh with shared nodes")
+    rng = np.random.default_rng(17)
+    nelem, npoin = 3000, 2000
+    connec = np.asfortranarray(np.array([rng.choice(npoin - 50, 4, replace=False) + 1
+                                         for _ in range(nelem)], dtype=np.int32))
+    coord = np.asfortranarray(rng.random((npoin, 3)))
+    t = m.tet_tables()
+    He = np.zeros((3, 3, 4, nelem), order="F")
+    gp = np.zeros((1, 4, nelem), order="F")
+    m.host_lib().msodh_elem_jacobian(nelem, npoin, 4, connec.ctypes.data, coord.ctypes.data,
+                                     t.dNgp.ctypes.data, t.wgp.ctypes.data, gp.ctypes.data,
+                                     He.ctypes.data)
+    f = lambda *sh: np.asfortranarray(rng.random(sh))
+    tp = m.TetProblem(t, nelem, npoin, connec, coord, He, gp, f(npoin, 3), f(npoin, 3),
+                      1 + f(npoin), 1 + f(npoin), 2 + f(npoin), 1 + f(npoin), f(npoin),
+                      f(nelem, 4), f(nelem, 4))
+    out = gpu_tet(tp, 1.1, 0.9)
+    want = oracle_tet(tp, 1.1, 0.9)
+    for key, got, w in zip(KEYS, out, want):
+        assert rel_l2(got, w) <= TOL, (key, rel_l2(got, w))
+        assert np.all(np.asarray(got)[npoin - 50:] == 0.0), key
+
+
 def test_tet_jacobian_matches_golden_and_host(gpu):
     from golden.make_golden import digest
     say("[gpu] TET geometry vs the reference's tet_jacobian")
