@@ -65,13 +65,31 @@ __device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t 
 template <int NN, int ME, int NT, bool CLOSURE, typename TG, typename TF>
 __device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64_t e0, TG (*sMu)[ME][NN])
 {
-    for (int i = threadIdx.x; i < ME * NN; i += NT) {
-        const int eb = i % ME, l = i / ME;
-        const int64_t e = e0 + eb;
-        if (e < a.nelem) {
-            const int64_t ge = e + a.nelem * (int64_t)a.tab.l2g[l];  // reference layout (nelem,ngaus)
-            sMu[0][eb][l] = a.mu_e[ge];
-            sMu[1][eb][l] = (!CLOSURE && a.mu_sgs) ? a.mu_sgs[ge] : (TG)0;  // NULL: closure fallback
+    // every load unconditional and issued before any LDS write (entries past
+    // the table or the mesh re-read a valid entry and are not written or not
+    // used), so that the loads of one thread are in flight together
+    constexpr int IT = (ME * NN + NT - 1) / NT;
+    int32_t gl[IT];
+    TG v0[IT], v1[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = (int)threadIdx.x + k * NT < ME * NN ? (int)threadIdx.x + k * NT : ME * NN - 1;
+        gl[k] = a.tab.l2g[i / ME];
+    }
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = (int)threadIdx.x + k * NT < ME * NN ? (int)threadIdx.x + k * NT : ME * NN - 1;
+        const int64_t e = e0 + i % ME < a.nelem ? e0 + i % ME : a.nelem - 1;
+        const int64_t ge = e + a.nelem * (int64_t)gl[k];  // reference layout (nelem,ngaus)
+        v0[k] = a.mu_e[ge];
+        v1[k] = (!CLOSURE && a.mu_sgs) ? a.mu_sgs[ge] : (TG)0;  // NULL: closure fallback
+    }
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < ME * NN) {
+            sMu[0][i % ME][i / ME] = v0[k];
+            sMu[1][i % ME][i / ME] = v1[k];
         }
     }
 }
@@ -124,15 +142,51 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
     const T Cp = (T)a.Cp, Pr = (T)a.Pr;
 
     DiffuNode<TG, TF> cur, nxt;
+    static_assert(Q * Q <= S::NT, "one derivative-table entry per thread");
     {
-        const int64_t e = b * S::BATCH + le;
-        if (le < EPB && e < a.nelem) diffu_fetch<NN, CLOSURE>(a, e, l, cur);
-    }
-    if (!kMuPerPass) diffu_stage_mu<NN, ME, NT, CLOSURE>(a, b * S::BATCH, sMu);
-    for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
-    for (uint32_t i = t; i < U; i += blockDim.x)
+        // prologue: the tables and the connectivity first, then the values
+        // that depend on them, then the LDS writes; every load unconditional
+        // (idle threads read a real entry), so that each group is in flight
+        // together
+        const T dv = (T)a.tab.D[t < Q * Q ? t : Q * Q - 1];
+        constexpr int IT = (ME * NN + NT - 1) / NT;
+        int32_t gl[IT];
+        TG v0[IT], v1[IT];
+        auto mu_i = [&](int k) {
+            return (int)t + k * NT < ME * NN ? (int)t + k * NT : ME * NN - 1;
+        };
+        if (!kMuPerPass) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
+            for (int k = 0; k < IT; ++k) gl[k] = a.tab.l2g[mu_i(k) / ME];
+        }
+        const int64_t e = b * S::BATCH + le;
+        const int64_t ec = e < a.nelem ? e : a.nelem - 1;
+        diffu_fetch<NN, CLOSURE>(a, ec, l, cur);
+        if (!kMuPerPass) {
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int i = mu_i(k);
+                const int64_t em = b * S::BATCH + i % ME < a.nelem ? b * S::BATCH + i % ME : a.nelem - 1;
+                const int64_t ge = em + a.nelem * (int64_t)gl[k];
+                v0[k] = a.mu_e[ge];
+                v1[k] = (!CLOSURE && a.mu_sgs) ? a.mu_sgs[ge] : (TG)0;
+            }
+        }
+        for (uint32_t i = t; i < U; i += blockDim.x)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
+        if (t < Q * Q) sD[t] = dv;
+        if (!kMuPerPass) {
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                const int i = (int)t + k * NT;
+                if (i < ME * NN) {
+                    sMu[0][i % ME][i / ME] = v0[k];
+                    sMu[1][i % ME][i / ME] = v1[k];
+                }
+            }
+        }
+    }
     if constexpr (CLOSURE) mlp_stage(smlp, mlp);
 
 #pragma unroll
