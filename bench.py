@@ -35,6 +35,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_convec_diffu_128cubed
 TRAFFIC_TAG = "traffic-json: "
 KERNEL_RE = {"full_convec_ijk": r"k_convec<(\d+), double, double>",
              "full_diffusion_ijk": r"k_diffu<(\d+), double, double, false>",
+             "fem_convec": r"k_fem_convec<true>", "fem_diffu": r"k_fem_diffu<true>",
              "zero_fill": r"msod::k_zero\("}
 
 
@@ -113,7 +114,8 @@ def summarize_pmc(fetch_csv, write_csv, n, porder):
     prob_n = (n * porder + 1) ** 3
     E = n ** 3 * (porder + 1) ** 3
     bc, bd = canonical_bytes(E, prob_n, 8, 8)
-    algo = {"full_convec_ijk": bc, "full_diffusion_ijk": bd}
+    tc, td = tet_bytes(prob_n // 4)
+    algo = {"full_convec_ijk": bc, "full_diffusion_ijk": bd, "fem_convec": tc, "fem_diffu": td}
     for name, ent in out["kernels"].items():
         if "fetch_size_bytes" in ent and "write_size_bytes" in ent:
             ent["traffic_bytes"] = ent["fetch_size_bytes"] + ent["write_size_bytes"]
@@ -271,7 +273,7 @@ def tet_bytes(nelem):
     return nelem * (336 + 4 * (72 + 40)), nelem * (400 + 4 * (48 + 40))
 
 
-def time_tet(prob, R, D, steps, warmup, nsample=2000):
+def time_tet(prob, R, D, steps, warmup, workload, nsample=2000):
     """fem_convec + fem_diffu on the reference driver's TET04 data over the
     workload's nodal fields (npoin/4 disjoint tetrahedra, tet_jacobian on the
     device), outputs in the workload's R, D; sampled elements against the oracle."""
@@ -329,6 +331,7 @@ def time_tet(prob, R, D, steps, warmup, nsample=2000):
             "bytes": {"fem_convec": bc, "fem_diffu": bd},
             "hbm_frac": {"fem_convec": bc / (cms * 1e-3) / HBM_PEAK,
                          "fem_diffu": bd / (dms * 1e-3) / HBM_PEAK},
+            "traffic": {k: load_traffic(*workload, k) for k in ("fem_convec", "fem_diffu")},
             "sampled_rel_l2": errs,
             "note": "TET04 kernels on npoin/4 disjoint tetrahedra over the workload's nodal "
                     "fields (main.f90:364-407), jittered vertices, random mu_e, mu_sgs"}
@@ -472,7 +475,7 @@ def main():
     tet = None
     if not args.no_tet:
         log("timing the TET04 kernels")
-        tet = time_tet(prob, R, D, args.steps, args.warmup)
+        tet = time_tet(prob, R, D, args.steps, args.warmup, (n, p))
         log(f"tet {tet}")
 
     split = {}
