@@ -13,7 +13,7 @@
 //     TARGET node is folded into the fluxes at the source nodes:
 //     U_d(s) = sum_i He(i,d)u_i(s).  Since He is taken at the target node,
 //       div(q (x) u)_i = sum_d D_d[ q_i U_d ],  div u = sum_d He(:,d).D_d[u], ...
-//     Per line the thread forms the plain 1-D derivatives of p, u, q and the
+//     Per line the thread forms the plain 1-D derivatives of r, p, H, rH, u, q and the
 //     derivatives of the products with U_d; the node's own r, u, q, H and He
 //     column then contract them (eleven accumulators over the three
 //     directions; the sweep code is shared by the directions).  LDS holds the
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
                     const T Ua = h0 * sF[le][l][4] + h1 * sF[le][l][5] + h2 * sF[le][l][6];
                     const T Qa = h0 * sF[le][l][7] + h1 * sF[le][l][8] + h2 * sF[le][l][9];
                     // 1-D derivatives along this line of the stored values
-                    T dp = 0, du[3] = {0, 0, 0}, dq[3] = {0, 0, 0};
+                    T dp = 0, du[3] = {0, 0, 0}, dq[3] = {0, 0, 0}, dr = 0, dH = 0, drH = 0;
 #pragma unroll 1
                     for (int n = 0; n < Q; ++n) {
                         const int s = base + n * stride;
@@ -203,9 +203,9 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
                         const T rs = sF[le][s][0], ps = sF[le][s][1], Hs = sF[le][s][2], rHs = sF[le][s][3];
                         const T us0 = sF[le][s][4], us1 = sF[le][s][5], us2 = sF[le][s][6];
                         const T qs0 = sF[le][s][7], qs1 = sF[le][s][8], qs2 = sF[le][s][9];
-                        const T UaD = Ua * Dv, QaD = Qa * Dv;
-                        sRho += UaD * rs;
-                        sE += QaD * Hs + UaD * rHs;
+                        dr += Dv * rs;
+                        dH += Dv * Hs;
+                        drH += Dv * rHs;
                         dp += Dv * ps;
                         du[0] += Dv * us0; du[1] += Dv * us1; du[2] += Dv * us2;
                         dq[0] += Dv * qs0; dq[1] += Dv * qs1; dq[2] += Dv * qs2;
@@ -216,6 +216,8 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
                         aM[1] += DU * (qs1 + r * us1);
                         aM[2] += DU * (qs2 + r * us2);
                     }
+                    sRho += Ua * dr;
+                    sE += Qa * dH + Ua * drH;
                     const T dp4 = (T)4 * dp;
                     mM[0] += Qa * du[0] + Ua * dq[0] + h0 * dp4;
                     mM[1] += Qa * du[1] + Ua * dq[1] + h1 * dp4;
