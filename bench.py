@@ -273,7 +273,37 @@ def tet_bytes(nelem):
     return nelem * (336 + 4 * (72 + 40)), nelem * (400 + 4 * (48 + 40))
 
 
-def time_tet(prob, R, D, steps, warmup, workload, nsample=2000):
+def tet_cpu_baseline(tp, nsub=1 << 20, nruns=3):
+    """The reference's own fem_convec and fem_diffu (oracle/_ref, compiled
+    from its sources, one core) on the first nsub tetrahedra of the mesh."""
+    from oracle_lib import RefTet, ref_available
+    if not ref_available(4):
+        return None
+    ref = RefTet(4)
+    n = min(nsub, tp.nelem)
+    f = np.asfortranarray
+    sub = m.TetProblem(tp.tables, n, 4 * n, f(tp.connec[:n]), f(tp.coord[:4 * n]),
+                       f(tp.He[:, :, :, :n]), f(tp.gpvol[:, :, :n]),
+                       *[f(getattr(tp, k)[:4 * n]) for k in ("u", "q", "rho", "pr", "E", "Tem", "mu_fluid")],
+                       f(tp.mu_e[:n]), f(tp.mu_sgs[:n]), Cp=tp.Cp, Pr=tp.Pr)
+    tc, td = [], []
+    for _ in range(nruns):
+        t0 = time.perf_counter()
+        ref.fem_convec(sub)
+        t1 = time.perf_counter()
+        ref.fem_diffu(sub)
+        t2 = time.perf_counter()
+        tc.append(t1 - t0)
+        td.append(t2 - t1)
+    tcm, tdm = float(np.median(tc)), float(np.median(td))
+    return dict(value=n / (tcm + tdm), unit="tetrahedra/s (fem_convec + fem_diffu)", cores=1,
+                kind="reference",
+                sample=f"first {n} tetrahedra of the same mesh, reference fem_convec + fem_diffu "
+                       f"(amdflang -O3, rp=8), median of {nruns}: convec {tcm*1e3:.1f} ms, "
+                       f"diffu {tdm*1e3:.1f} ms")
+
+
+def time_tet(prob, R, D, steps, warmup, workload, nsample=2000, with_cpu=True):
     """fem_convec + fem_diffu on the reference driver's TET04 data over the
     workload's nodal fields (npoin/4 disjoint tetrahedra, tet_jacobian on the
     device), outputs in the workload's R, D; sampled elements against the oracle."""
@@ -325,14 +355,17 @@ def time_tet(prob, R, D, steps, warmup, workload, nsample=2000):
     want = o.fem_convec(sub) + o.fem_diffu(sub)
     errs = {k: rel_l2(g[nodes], w) for k, g, w in
             zip(("Rmass", "Rmom", "Rener", "Dmass", "Dmom", "Dener"), R + D, want)}
+    cpu = tet_cpu_baseline(tp) if with_cpu else None
     return {"nelem": tp.nelem, "npoin": tp.npoin, "nbad": nbad, "tet_jacobian_ms": jac_ms,
             "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "tetrahedra_per_s": tp.nelem / ((cms + dms) * 1e-3),
             "kernels_ms": {"fem_convec": cms, "fem_diffu": dms, "zero_fill": zms},
             "bytes": {"fem_convec": bc, "fem_diffu": bd},
             "hbm_frac": {"fem_convec": bc / (cms * 1e-3) / HBM_PEAK,
                          "fem_diffu": bd / (dms * 1e-3) / HBM_PEAK},
             "traffic": {k: load_traffic(*workload, k) for k in ("fem_convec", "fem_diffu")},
             "sampled_rel_l2": errs,
+            "cpu_baseline": cpu,
             "note": "TET04 kernels on npoin/4 disjoint tetrahedra over the workload's nodal "
                     "fields (main.f90:364-407), jittered vertices, random mu_e, mu_sgs"}
 
@@ -475,7 +508,7 @@ def main():
     tet = None
     if not args.no_tet:
         log("timing the TET04 kernels")
-        tet = time_tet(prob, R, D, args.steps, args.warmup, (n, p))
+        tet = time_tet(prob, R, D, args.steps, args.warmup, (n, p), with_cpu=not args.no_cpu_baseline)
         log(f"tet {tet}")
 
     split = {}
