@@ -152,19 +152,21 @@ def minimal_flops(E, q):
 
 TIMERS = {"fp64": (m.TIMER_CONVEC, m.TIMER_DIFFU),
           "mixed": (m.TIMER_CONVEC_MIXED, m.TIMER_DIFFU_MIXED),
-          "geo32": (m.TIMER_CONVEC_GEO32, m.TIMER_DIFFU_GEO32)}
+          "geo32": (m.TIMER_CONVEC_GEO32, m.TIMER_DIFFU_GEO32),
+          "fp32": (m.TIMER_CONVEC_FP32, m.TIMER_DIFFU_FP32)}
 
 
 def run_pair(prob, R, D, variant, f32=None):
     t = prob.tables
-    if variant == "mixed":
-        m.full_convec_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
-                                t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, f32["u"],
-                                f32["q"], f32["rho"], f32["E"], prob.gamma, *R)
-        m.full_diffusion_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
-                                   t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp,
-                                   prob.Pr, f32["rho"], f32["u"], f32["Tem"], f32["mu_fluid"],
-                                   f32["mu_e"], f32["mu_sgs"], *D)
+    if variant in ("mixed", "fp32"):
+        conv = m.full_convec_ijk_fp32 if variant == "fp32" else m.full_convec_ijk_mixed
+        diff = m.full_diffusion_ijk_fp32 if variant == "fp32" else m.full_diffusion_ijk_mixed
+        conv(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"], t.dlxigp_ip,
+             t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, f32["u"], f32["q"], f32["rho"], f32["E"],
+             prob.gamma, *R)
+        diff(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"], t.dlxigp_ip,
+             t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp, prob.Pr, f32["rho"], f32["u"],
+             f32["Tem"], f32["mu_fluid"], f32["mu_e"], f32["mu_sgs"], *D)
     elif variant == "geo32":
         m.full_convec_ijk_geo32(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
                                 t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.u, prob.q,
@@ -515,7 +517,7 @@ def main():
     if not args.no_mixed:
         f32 = prob.fp32()
         m.copyin(*[f32[k] for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")])
-        for variant in ("mixed", "geo32"):
+        for variant in ("mixed", "geo32", "fp32"):
             log(f"timing {variant}")
             r = timed(prob, R, D, args.steps, args.warmup, variant, f32=f32)
             log(f"{variant} {r}")
@@ -563,9 +565,11 @@ def main():
     notes = {"mixed": "every input stored in FP32, FP64 arithmetic and assembly; error = FP32 "
                       "rounding of the inputs, amplified by the derivative sweeps",
              "geo32": "He, gpvol, mu_e, mu_sgs stored in FP32, nodal fields FP64, FP64 "
-                      "arithmetic and assembly"}
+                      "arithmetic and assembly",
+             "fp32": "every input stored in FP32, FP32 gathers, fluxes and derivative sweeps; "
+                     "FP64 node sums, atomic assembly and pressure recovery"}
     for variant, r in split.items():
-        wb = (4, 4, 4) if variant == "mixed" else (4, 8, 4)
+        wb = (4, 8, 4) if variant == "geo32" else (4, 4, 4)
         bcm, bdm = canonical_bytes(E, N, *wb)
         rec[variant] = {"value": N / (r["ms_per_step"] * 1e-3), "ms_per_step": r["ms_per_step"],
                         "speedup_vs_fp64": r64["ms_per_step"] / r["ms_per_step"],

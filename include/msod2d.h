@@ -94,7 +94,8 @@ int msod_timing_enable(int enable);
  *         generator (msod_box_mesh, msod_flow_fields,
  *         msod_viscosity_fields), 11 TET convec (msod_fem_convec),
  *         12 TET diffu (msod_fem_diffu), 13 TET geometry
- *         (msod_tet_jacobian).  Returns accumulated ms and launch count. */
+ *         (msod_tet_jacobian), 14 convec fp32, 15 diffu fp32.  Returns
+ *         accumulated ms and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
 
@@ -209,6 +210,24 @@ int msod_full_diffusion_ijk_geo32(int porder, int nelem, int npoin, const int32_
                                   double Pr, const double *rho, const double *u, const double *Tem,
                                   const double *mu_fluid, const float *mu_e, const float *mu_sgs,
                                   double *Rmass, double *Rmom, double *Rener);
+/*   _fp32: the inputs of _mixed, with the gathers, fluxes and derivative
+ *     sweeps in FP32 arithmetic as well; the node sums, the assembly and the
+ *     pressure recovery stay FP64.  The error adds FP32 rounding inside the
+ *     sweeps (with their cancellation) to the rounding of the inputs; the
+ *     values are reported beside the timings. */
+int msod_full_convec_ijk_fp32(int porder, int nelem, int npoin, const int32_t *connec,
+                              const float *He, const float *gpvol, const double *dlxigp_ip,
+                              const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                              const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const float *u,
+                              const float *q, const float *rho, const float *E, double gamma_gas,
+                              double *Rmass, double *Rmom, double *Rener);
+int msod_full_diffusion_ijk_fp32(int porder, int nelem, int npoin, const int32_t *connec,
+                                 const float *He, const float *gpvol, const double *dlxigp_ip,
+                                 const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                                 const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                                 double Pr, const float *rho, const float *u, const float *Tem,
+                                 const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                 double *Rmass, double *Rmom, double *Rener);
 
 /* ---- learned subgrid-viscosity closure ----------------------------------
  * No counterpart in the reference: its driver sets mu_sgs = 1

@@ -41,7 +41,8 @@ MSOD_ERR_MODEL = 8
 # timer ids of msod_timing_get
 (TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP,
  TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32, TIMER_GEOMETRY,
- TIMER_GENERATOR, TIMER_FEM_CONVEC, TIMER_FEM_DIFFU, TIMER_TET_GEOMETRY) = range(14)
+ TIMER_GENERATOR, TIMER_FEM_CONVEC, TIMER_FEM_DIFFU, TIMER_TET_GEOMETRY, TIMER_CONVEC_FP32,
+ TIMER_DIFFU_FP32) = range(16)
 
 # the C-ABI entry points declared in include/msod2d.h (checked by the tests)
 ABI_SYMBOLS = (
@@ -57,6 +58,7 @@ ABI_SYMBOLS = (
     "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
     "msod_closure_loss", "msod_elem_jacobian", "msod_box_mesh", "msod_flow_fields",
     "msod_viscosity_fields", "msod_tet_jacobian", "msod_fem_convec", "msod_fem_diffu",
+    "msod_full_convec_ijk_fp32", "msod_full_diffusion_ijk_fp32",
 )
 
 
@@ -104,6 +106,8 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_full_convec_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 12 + [_d] + [_vp] * 3
         L.msod_full_diffusion_ijk_mixed.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
         L.msod_full_convec_ijk_geo32.argtypes = [_i, _i, _i] + [_vp] * 16
+        L.msod_full_convec_ijk_fp32.argtypes = [_i, _i, _i] + [_vp] * 12 + [_d] + [_vp] * 3
+        L.msod_full_diffusion_ijk_fp32.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
         L.msod_full_diffusion_ijk_geo32.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
         L.msod_full_diffusion_ijk_closure.argtypes = ([_i, _i, _i] + [_vp] * 11 + [_d, _d] +
                                                       [_vp] * 10)
@@ -320,6 +324,27 @@ def full_diffusion_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, inv
                              mu_sgs32, Rmass, Rmom, Rener):
     """Diffusive residual from FP32 state and geometry (FP64 arithmetic and assembly)."""
     _check(hip_lib().msod_full_diffusion_ijk_mixed(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
+        float(Cp), float(Pr),
+        *map(_ptr, (rho32, u32, Tem32, mu_fluid32, mu_e32, mu_sgs32, Rmass, Rmom, Rener))))
+
+
+def full_convec_ijk_fp32(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                         gmshAtoJ, gmshAtoK, u32, q32, rho32, E32, gamma_gas, Rmass, Rmom, Rener):
+    """Convective residual with FP32 storage and FP32 gathers, fluxes and sweeps
+    (FP64 node sums, assembly and pressure recovery)."""
+    _check(hip_lib().msod_full_convec_ijk_fp32(
+        _porder(invAtoIJK), nelem, npoin,
+        *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
+                    u32, q32, rho32, E32)), float(gamma_gas), *map(_ptr, (Rmass, Rmom, Rener))))
+
+
+def full_diffusion_ijk_fp32(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                            gmshAtoJ, gmshAtoK, Cp, Pr, rho32, u32, Tem32, mu_fluid32, mu_e32,
+                            mu_sgs32, Rmass, Rmom, Rener):
+    """Diffusive residual with FP32 storage and FP32 arithmetic (FP64 node sums and assembly)."""
+    _check(hip_lib().msod_full_diffusion_ijk_fp32(
         _porder(invAtoIJK), nelem, npoin,
         *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
         float(Cp), float(Pr),

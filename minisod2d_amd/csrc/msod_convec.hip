@@ -59,12 +59,14 @@ __device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_
 }
 
 // Launch bound: 3 workgroups per CU (up to 168 VGPRs), room for the loads in
-// flight; 4 would force spills.
-template <int P, typename TG, typename TF>
+// flight; 4 would force spills.  TA: arithmetic type of the gathers, fluxes
+// and sweeps (FP64, or FP32 for the FP32-arithmetic form); the node sums and
+// the assembly stay FP64.
+template <int P, typename TG, typename TF, typename TA = double>
 __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a)
 {
     using S = Shape<P>;
-    using T = double;  // arithmetic
+    using T = TA;  // arithmetic
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX, NPASS = S::NPASS;
     constexpr int NF = 10;
     __shared__ T sD[Q * Q];
@@ -141,9 +143,11 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
                 const T r = cur.r, Ev = cur.E;
                 const T ux = cur.u[0], uy = cur.u[1], uz = cur.u[2];
                 const T k = ux * ux * (T)0.5 + uy * uy * (T)0.5 + uz * uz * (T)0.5;
-                // p = (gamma-1)(E - rho|u|^2/2) when no pressure array is given
-                const T pv = a.pr ? (T)cur.p
-                                  : a.gamma_m1 * (Ev - r * (T)0.5 * (ux * ux + uy * uy + uz * uz));
+                // p = (gamma-1)(E - rho|u|^2/2) when no pressure array is given, in
+                // FP64 whatever T: the difference cancels
+                const double p64 = a.gamma_m1 * ((double)Ev - (double)r * 0.5 *
+                                                     ((double)ux * ux + (double)uy * uy + (double)uz * uz));
+                const T pv = a.pr ? (T)cur.p : (T)p64;
                 const T H = Ev + k;
                 const T rH = r * Ev + r * k;
                 sF[le][l][0] = r;
@@ -248,11 +252,11 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
     }
 }
 
-template <int P, typename TG, typename TF>
+template <int P, typename TG, typename TF, typename TA = double>
 static hipError_t launch_p(const ConvecArgs<TG, TF> &a, hipStream_t s)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_convec<P, TG, TF>), dim3(plan_blocks(a.plan)), dim3(Shape<P>::NT), 0, s, a);
+    hipLaunchKernelGGL((k_convec<P, TG, TF, TA>), dim3(plan_blocks(a.plan)), dim3(Shape<P>::NT), 0, s, a);
     return hipGetLastError();
 }
 
@@ -268,6 +272,21 @@ hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s)
     case 6: return launch_p<6, TG, TF>(a, s);
     case 7: return launch_p<7, TG, TF>(a, s);
     case 8: return launch_p<8, TG, TF>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_convec_fp32(int porder, const ConvecArgs<float, float> &a, hipStream_t s)
+{
+    switch (porder) {
+    case 1: return launch_p<1, float, float, float>(a, s);
+    case 2: return launch_p<2, float, float, float>(a, s);
+    case 3: return launch_p<3, float, float, float>(a, s);
+    case 4: return launch_p<4, float, float, float>(a, s);
+    case 5: return launch_p<5, float, float, float>(a, s);
+    case 6: return launch_p<6, float, float, float>(a, s);
+    case 7: return launch_p<7, float, float, float>(a, s);
+    case 8: return launch_p<8, float, float, float>(a, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -97,11 +97,11 @@ __device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64
 // The closure form is bounded to 3 workgroups per CU: the compiler then keeps
 // the MFMA accumulators in VGPRs (about 148 in all) instead of 188 VGPRs and
 // 32 AGPRs, 3 waves per SIMD instead of 2 (4.7 against 6.2 ms at 96^3 p=3).
-template <int P, typename TG, typename TF, bool CLOSURE>
+template <int P, typename TG, typename TF, bool CLOSURE, typename TA = double>
 __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ? 3 : 1) k_diffu(DiffuArgs<TG, TF> a, const float *mlp)
 {
     using S = Shape<P>;
-    using T = double;  // arithmetic
+    using T = TA;  // arithmetic: FP64, or FP32 for the FP32-arithmetic form (node sums stay FP64)
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
     // sA (pass 1: rho, T, u) and sG (pass 2: fluxes of one direction) share
     // one buffer; so does the MLP scratch of the closure between the two
@@ -328,11 +328,11 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
     write_batch_nodes<UMAX, NT>(pl.mode, wn, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 
-template <int P, typename TG, typename TF, bool CLOSURE = false>
+template <int P, typename TG, typename TF, bool CLOSURE = false, typename TA = double>
 static hipError_t launch_p(const DiffuArgs<TG, TF> &a, hipStream_t s, const float *mlp = nullptr)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_diffu<P, TG, TF, CLOSURE>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s,
+    hipLaunchKernelGGL((k_diffu<P, TG, TF, CLOSURE, TA>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s,
                        a, mlp);
     return hipGetLastError();
 }
@@ -349,6 +349,21 @@ hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s)
     case 6: return launch_p<6, TG, TF>(a, s);
     case 7: return launch_p<7, TG, TF>(a, s);
     case 8: return launch_p<8, TG, TF>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_diffu_fp32(int porder, const DiffuArgs<float, float> &a, hipStream_t s)
+{
+    switch (porder) {
+    case 1: return launch_p<1, float, float, false, float>(a, s);
+    case 2: return launch_p<2, float, float, false, float>(a, s);
+    case 3: return launch_p<3, float, float, false, float>(a, s);
+    case 4: return launch_p<4, float, float, false, float>(a, s);
+    case 5: return launch_p<5, float, float, false, float>(a, s);
+    case 6: return launch_p<6, float, float, false, float>(a, s);
+    case 7: return launch_p<7, float, float, false, float>(a, s);
+    case 8: return launch_p<8, float, float, false, float>(a, s);
     default: return hipErrorInvalidValue;
     }
 }

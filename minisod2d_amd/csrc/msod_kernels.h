@@ -115,6 +115,10 @@ template <typename TG, typename TF>
 hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s);
 template <typename TG, typename TF>
 hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s);
+// FP32 storage and FP32 gathers, fluxes and derivative sweeps; FP64 node sums,
+// assembly and (convection) pressure recovery.
+hipError_t launch_convec_fp32(int porder, const ConvecArgs<float, float> &a, hipStream_t s);
+hipError_t launch_diffu_fp32(int porder, const DiffuArgs<float, float> &a, hipStream_t s);
 
 // Zero Rmass (n), Rmom (3n), Rener (n).
 hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64_t n,

@@ -228,7 +228,7 @@ struct CheckedMesh {
     }
 };
 
-constexpr int kNumTimers = 14;
+constexpr int kNumTimers = 16;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -668,7 +668,7 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
                   const TG *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
                   const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, const TF *u,
                   const TF *q, const TF *rho, const TF *pr, const TF *E, double gamma_gas,
-                  double *Rmass, double *Rmom, double *Rener, int timer)
+                  double *Rmass, double *Rmom, double *Rener, int timer, bool fp32_arith = false)
 {
     NEED_INIT();
     int st = check_sizes(porder, nelem, npoin);
@@ -711,7 +711,11 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     a.Rener = (double *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
     if ((st = run_assembled(a, *plan, timer,
-                            [&](const ConvecArgs<TG, TF> &x) { return launch_convec<TG, TF>(porder, x, g.stream); },
+                            [&](const ConvecArgs<TG, TF> &x) {
+                                if constexpr (std::is_same<TG, float>::value && std::is_same<TF, float>::value)
+                                    if (fp32_arith) return launch_convec_fp32(porder, x, g.stream);
+                                return launch_convec<TG, TF>(porder, x, g.stream);
+                            },
                             "convection kernel launch", kConvecNatural)))
         return st;
     return finish_call("convection kernel");
@@ -723,7 +727,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, double Cp,
                  double Pr, const TF *rho, const TF *u, const TF *Tem, const TF *mu_fluid,
                  const TG *mu_e, const TG *mu_sgs, double *Rmass, double *Rmom, double *Rener,
-                 int timer, bool closure = false, const float *dmlp = nullptr)
+                 int timer, bool closure = false, const float *dmlp = nullptr, bool fp32_arith = false)
 {
     NEED_INIT();
     int st = check_sizes(porder, nelem, npoin);
@@ -774,6 +778,8 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
                             [&](const DiffuArgs<TG, TF> &x) {
                                 if constexpr (std::is_same<TG, double>::value && std::is_same<TF, double>::value)
                                     if (dmlp) return launch_diffu_closure(porder, x, dmlp, g.stream);
+                                if constexpr (std::is_same<TG, float>::value && std::is_same<TF, float>::value)
+                                    if (fp32_arith) return launch_diffu_fp32(porder, x, g.stream);
                                 return launch_diffu<TG, TF>(porder, x, g.stream);
                             },
                             "diffusion kernel launch", kDiffuNatural)))
@@ -1698,6 +1704,34 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
     return diffu_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                mu_sgs, Rmass, Rmom, Rener, 4);
+}
+
+int msod_full_convec_ijk_fp32(int porder, int nelem, int npoin, const int32_t *connec,
+                              const float *He, const float *gpvol, const double *dlxigp_ip,
+                              const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                              const int32_t *gmshAtoJ, const int32_t *gmshAtoK, const float *u,
+                              const float *q, const float *rho, const float *E, double gamma_gas,
+                              double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
+    return convec_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
+                                Rmass, Rmom, Rener, 14, true);
+}
+
+int msod_full_diffusion_ijk_fp32(int porder, int nelem, int npoin, const int32_t *connec,
+                                 const float *He, const float *gpvol, const double *dlxigp_ip,
+                                 const int32_t *invAtoIJK, const int32_t *gmshAtoI,
+                                 const int32_t *gmshAtoJ, const int32_t *gmshAtoK, double Cp,
+                                 double Pr, const float *rho, const float *u, const float *Tem,
+                                 const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                 double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    return diffu_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                               gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
+                               mu_sgs, Rmass, Rmom, Rener, 15, false, nullptr, true);
 }
 
 int msod_full_convec_ijk_geo32(int porder, int nelem, int npoin, const int32_t *connec,

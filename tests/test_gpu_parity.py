@@ -29,7 +29,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def gpu_residuals(prob, mixed=False):
+def gpu_residuals(prob, mixed=False, fp32=False):
     """Both kernels on the GPU; returns (Rmass, Rmom, Rener, Dmass, Dmom, Dener)."""
     R = prob.outputs()
     D = prob.outputs()
@@ -44,13 +44,13 @@ def gpu_residuals(prob, mixed=False):
     m.create(*R, *D)
     try:
         if mixed:
-            m.full_convec_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"],
-                                    t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, f["u"],
-                                    f["q"], f["rho"], f["E"], prob.gamma, *R)
-            m.full_diffusion_ijk_mixed(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"],
-                                       t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp,
-                                       prob.Pr, f["rho"], f["u"], f["Tem"], f["mu_fluid"],
-                                       f["mu_e"], f["mu_sgs"], *D)
+            conv = m.full_convec_ijk_fp32 if fp32 else m.full_convec_ijk_mixed
+            diff = m.full_diffusion_ijk_fp32 if fp32 else m.full_diffusion_ijk_mixed
+            conv(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip, t.invAtoIJK,
+                 t.AtoI, t.AtoJ, t.AtoK, f["u"], f["q"], f["rho"], f["E"], prob.gamma, *R)
+            diff(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip, t.invAtoIJK,
+                 t.AtoI, t.AtoJ, t.AtoK, prob.Cp, prob.Pr, f["rho"], f["u"], f["Tem"],
+                 f["mu_fluid"], f["mu_e"], f["mu_sgs"], *D)
         else:
             m.full_convec_ijk(*prob.convec_args(*R))
             m.full_diffusion_ijk(*prob.diffu_args(*D))
@@ -140,6 +140,32 @@ def test_mixed_precision_error(gpu, p, n):
     for k in KEYS:
         assert kern[k] <= TOL64, (k, kern[k])
         assert e2e[k] <= 4 * TOL_MIXED, (k, e2e[k])
+
+
+TOL_FP32 = 1e-4  # stated bar of the FP32-arithmetic form at these sizes
+
+
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (7, 3)])
+def test_fp32_arithmetic_error(gpu, p, n):
+    """FP32 storage and FP32 gathers, fluxes and derivative sweeps, FP64 node
+    sums and assembly: the error against the FP64 oracle (on the FP32-rounded
+    inputs, and on the FP64 inputs) is printed and stays within TOL_FP32; it
+    differs from the FP64-arithmetic form by more than rounding of the sums."""
+    say(f"[gpu] FP32 arithmetic p={p} n={n}")
+    prob = m.box_problem(n, p, jitter=0.05)
+    o = Oracle()
+    ref = o.convec(prob, 4) + o.diffu(prob, 4)
+    q32 = rounded_to_fp32(prob)
+    ref32 = o.convec(q32, 4) + o.diffu(q32, 4)
+    out = gpu_residuals(prob, mixed=True, fp32=True)
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, out, ref)}
+    kern = {k: rel_l2(g, w) for k, g, w in zip(KEYS, out, ref32)}
+    say("    vs FP64 oracle on FP64 inputs: " + " ".join(f"{k}={v:.2e}" for k, v in e2e.items()))
+    say("    vs FP64 oracle on the FP32 inputs: " + " ".join(f"{k}={v:.2e}" for k, v in kern.items()))
+    for k in KEYS:
+        assert kern[k] <= TOL_FP32, (k, kern[k])
+        assert e2e[k] <= TOL_FP32, (k, e2e[k])
+    assert max(kern.values()) > 1e-10  # the sweeps really ran in FP32
 
 
 def gpu_residuals_geo32(prob):
