@@ -33,8 +33,8 @@ FP64_PEAK = 78.6e12  # MI355X vector FP64, flop/s
 CALIB_BYTES = 2 << 30  # device-to-device copy that scales the HBM byte counters
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_convec_diffu_128cubed_p4.md")
 TRAFFIC_TAG = "traffic-json: "
-KERNEL_RE = {"full_convec_ijk": r"k_convec<(\d+), double, double>",
-             "full_diffusion_ijk": r"k_diffu<(\d+), double, double, false>",
+KERNEL_RE = {"full_convec_ijk": r"k_convec<(\d+), double, double(, double)?>",
+             "full_diffusion_ijk": r"k_diffu<(\d+), double, double, false(, double)?>",
              "fem_convec": r"k_fem_convec<true>", "fem_diffu": r"k_fem_diffu<true>",
              "zero_fill": r"msod::k_zero\("}
 
