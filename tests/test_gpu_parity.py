@@ -296,6 +296,14 @@ def test_error_paths(gpu):
             assert e.value.status == m.MSOD_ERR_MESH
         finally:
             m.delete(small)
+        # gamma <= 1 for the forms that recover the pressure
+        f = prob.fp32()
+        t = prob.tables
+        for fn in (L.msod_full_convec_ijk_mixed, L.msod_full_convec_ijk_fp32):
+            st = fn(3, prob.nelem, prob.npoin, *[m._ptr(x) for x in (
+                prob.connec, f["He"], f["gpvol"], t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK,
+                f["u"], f["q"], f["rho"], f["E"])], 1.0, *[m._ptr(x) for x in R])
+            assert st == m.MSOD_ERR_ARG
         # the valid call still works after the failures
         m.full_convec_ijk(*prob.convec_args(*R))
         m.update_host(*R)

@@ -186,6 +186,16 @@ def test_bad_connectivity_is_rejected(gpu):
         assert L.msod_fem_convec(0, tp.npoin, *[m._ptr(a) for a in (
             tp.connec, tp.tables.Ngp, tp.tables.dNgp, tp.He, tp.gpvol, tp.u, tp.q, tp.rho, tp.pr,
             tp.E, *R)]) == m.MSOD_ERR_ARG
+        # Pr = 0, a missing table, a missing input
+        assert L.msod_fem_diffu(tp.nelem, tp.npoin, *[m._ptr(a) for a in (
+            tp.connec, tp.tables.Ngp, tp.tables.dNgp, tp.He, tp.gpvol)], 1.0, 0.0, *[m._ptr(a) for a in (
+                tp.rho, tp.u, tp.Tem, tp.mu_fluid, tp.mu_e, tp.mu_sgs, *R)]) == m.MSOD_ERR_ARG
+        assert L.msod_fem_convec(tp.nelem, tp.npoin, *[m._ptr(a) for a in (
+            tp.connec, None, tp.tables.dNgp, tp.He, tp.gpvol, tp.u, tp.q, tp.rho, tp.pr,
+            tp.E, *R)]) == m.MSOD_ERR_ARG
+        assert L.msod_fem_convec(tp.nelem, tp.npoin, *[m._ptr(a) for a in (
+            tp.connec, tp.tables.Ngp, tp.tables.dNgp, tp.He, tp.gpvol, tp.u, tp.q, tp.rho, None,
+            tp.E, *R)]) == m.MSOD_ERR_ARG
     finally:
         m.delete(*ins, *R)
 
