@@ -31,7 +31,7 @@ namespace msod {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kHeStride = 37;  // doubles per element in LDS (odd: no bank pile-up)
+constexpr int kHeStride = 37;  // doubles per element in LDS (odd: spreads the elements over the banks)
 
 __device__ __forceinline__ int64_t node_of(const int32_t *connec, int64_t nelem, int64_t npoin, int64_t e,
                                            int a)
