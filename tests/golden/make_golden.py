@@ -20,7 +20,12 @@ tell a generator change from a kernel change.  Fixtures:
   tet_jac.npz     He, gpvol of tet_jacobian on a jittered tetrahedral mesh
   tet_case_*.npz  Rmass/Rmom/Rener of fem_convec, Dmass/Dmom/Dener of fem_diffu
 
-`python tests/golden/make_golden.py --tet` writes the TET fixtures only.
+  digest_*.npz    the reference's residuals at configurations too large to
+                  commit whole (64^3 p=4, 48^3 p=7, 96^3 p=3): per-component
+                  L2 norms and sums, and the values at 4096 seeded nodes
+
+`python tests/golden/make_golden.py --tet` writes the TET fixtures only,
+`--digests` the digests only.
 """
 import hashlib
 import json
@@ -63,6 +68,39 @@ TET_CASES = {
 TET_INPUT_FIELDS = ("connec", "coord", "He", "gpvol", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid",
                     "mu_e", "mu_sgs")
 
+# (name, builder) -- the full-size configurations, as the GPU tests build them
+DIGEST_CASES = {
+    "digest_cfg1_64cubed_p4": lambda: m.box_problem(64, 4, jitter=0.05, threads=8),
+    "digest_cfg3_48cubed_p7": lambda: m.box_problem(48, 7, jitter=0.05, rho_jump=0.1, threads=8),
+    "digest_cfg4_96cubed_p3": lambda: m.box_problem(96, 3, jitter=0.05, threads=8),
+}
+DIGEST_NODES = 4096
+DIGEST_SEED = 4096
+
+
+def digest_nodes(npoin):
+    """The 0-based nodes whose values a digest keeps."""
+    return np.sort(np.random.default_rng(DIGEST_SEED).choice(npoin, size=DIGEST_NODES, replace=False))
+
+
+# digest columns of each residual array (Rmass, Rmom, Rener, Dmass, Dmom, Dener):
+# the parity bar is the relative L2 error of each array
+DIGEST_GROUPS = (("Rmass", (0,)), ("Rmom", (1, 2, 3)), ("Rener", (4,)), ("Dmass", (5,)),
+                 ("Dmom", (6, 7, 8)), ("Dener", (9,)))
+
+
+def residual_digest(outputs, npoin):
+    """outputs: Rmass, Rmom, Rener, Dmass, Dmom, Dener -> columns of 10 components."""
+    cols = []
+    for a in outputs:
+        a = np.asarray(a).reshape(npoin, -1, order="F")
+        cols.extend(a[:, j] for j in range(a.shape[1]))
+    idx = digest_nodes(npoin)
+    return dict(nodes=idx, values=np.stack([c[idx] for c in cols], axis=1),
+                norms=np.array([np.linalg.norm(c) for c in cols]),
+                sums=np.array([c.sum() for c in cols]))
+
+
 INPUT_FIELDS = ("connec", "He", "gpvol", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")
 
 
@@ -101,9 +139,25 @@ def write_tet():
         print(name, tp.nelem, tp.npoin, "ok")
 
 
+def write_digests():
+    for name, build in DIGEST_CASES.items():
+        prob = build()
+        ref = RefLib(prob.porder)
+        Rmass, Rmom, Rener, _ = ref.convec(prob)
+        Dmass, Dmom, Dener = ref.diffu(prob)
+        dg = residual_digest((Rmass, Rmom, Rener, Dmass, Dmom, Dener), prob.npoin)
+        meta = dict(prob.meta, nelem=prob.nelem, npoin=prob.npoin, inputs=input_digests(prob),
+                    components="Rmass Rmom1 Rmom2 Rmom3 Rener Dmass Dmom1 Dmom2 Dmom3 Dener")
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **dg, meta=json.dumps(meta))
+        print(name, prob.nelem, prob.npoin, "ok", flush=True)
+
+
 def main():
     if "--tet" in sys.argv[1:]:
         write_tet()
+        return
+    if "--digests" in sys.argv[1:]:
+        write_digests()
         return
     write_tet()
     for p in range(2, 9):

@@ -4,13 +4,16 @@ CPU oracle cannot run the whole mesh in seconds:
   128^3 porder 4 (the benchmark workload), 48^3 porder 7 with the 10%
   density jump, 96^3 porder 3 (the closure configuration).
 
-Two checks per mesh:
+Checks per mesh:
   * sampled nodes: the residuals at 2000 random nodes against the oracle run
     on exactly the elements that touch them (so each sampled node receives
     all of its contributions), relative L2 <= 1e-12;
   * a uniform state (constant rho, u, p, T on the same jittered mesh): every
     residual vanishes, to rounding, relative to the residuals of the
-    non-uniform state.
+    non-uniform state;
+  * where the reference's own residuals were digested (tests/golden/digest_*:
+    48^3 p=7, 96^3 p=3), the values at its 4096 seeded nodes and the
+    per-component L2 norms, relative error <= 1e-12.
 """
 import os
 import sys
@@ -30,6 +33,27 @@ CONFIGS = [
     (48, 7, 0.05, 0.1),   # porder 7, density jump
     (96, 3, 0.05, 0.0),   # closure configuration
 ]
+DIGEST = {(48, 7): "digest_cfg3_48cubed_p7", (96, 3): "digest_cfg4_96cubed_p3"}
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def check_reference_digest(name, prob, out):
+    """GPU residuals against the reference's digest of the same inputs."""
+    import json
+    from golden.make_golden import DIGEST_GROUPS, input_digests, residual_digest
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    meta = json.loads(str(z["meta"]))
+    assert input_digests(prob) == meta["inputs"], "generator output changed"
+    dg = residual_digest(out, prob.npoin)
+    assert np.array_equal(dg["nodes"], z["nodes"])
+    # per residual array, like every parity check (a component such as Rmom3 of
+    # these fields is mostly cancellation, so its own relative error measures
+    # the summation order, not the residual)
+    for key, cols in DIGEST_GROUPS:
+        err = rel_l2(dg["values"][:, cols], z["values"][:, cols])
+        assert err <= 1e-12, (key, err)
+        ng, nz = np.linalg.norm(dg["norms"][list(cols)]), np.linalg.norm(z["norms"][list(cols)])
+        assert abs(ng - nz) <= 1e-12 * nz, key
 
 
 def say(*a):
@@ -100,6 +124,9 @@ def test_full_size_sampled_parity_and_uniform_state(gpu, n, p, jitter, jump):
     for key, got, w in zip(KEYS, out, want):
         err = rel_l2(got[nodes - 1], w)
         assert err <= 1e-12, (key, err)
+    if (n, p) in DIGEST:
+        say(f"[gpu] full size {n}^3 p={p}: against the reference's digest")
+        check_reference_digest(DIGEST[(n, p)], prob, out)
     scale = [float(np.sqrt(np.mean(np.square(r)))) for r in out]
     del out
     say(f"[gpu] full size {n}^3 p={p}: uniform state")

@@ -107,6 +107,9 @@ def test_fp64_config1_64cubed_p4_jittered(gpu):
     # size-independent property: the weak divergence of tau sums to zero
     for r in range(3):
         assert abs(out[4][:, r].sum()) <= 1e-12 * np.abs(out[4][:, r]).sum()
+    # the reference's own residuals of the same inputs (digest: 4096 seeded nodes, norms)
+    from test_gpu_full_configs import check_reference_digest
+    check_reference_digest("digest_cfg1_64cubed_p4", prob, out)
 
 
 def rounded_to_fp32(prob):
