@@ -43,6 +43,32 @@ __global__ void k_zero(double *p, int64_t n)
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) p[n - 1] = 0.0;
 }
 
+// The three outputs in one launch (each 16-byte aligned): thread i stores one
+// 16-byte zero into the array whose range holds i.  Used below 16 M doubles,
+// where launches dominate (8^3 p=3: 0.109 -> 0.101 ms per pair); on the
+// 128^3 p=4 outputs three launches of k_zero measured 0.78 against 0.83 ms.
+struct ZeroSet {
+    double *p[3];
+    int64_t n[3];
+};
+
+__global__ void k_zero3(ZeroSet z)
+{
+    const int64_t a = z.n[0] / 2, b = z.n[1] / 2, c = z.n[2] / 2;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double2 *q = nullptr;
+    if (i < a)
+        q = reinterpret_cast<double2 *>(z.p[0]) + i;
+    else if (i < a + b)
+        q = reinterpret_cast<double2 *>(z.p[1]) + (i - a);
+    else if (i < a + b + c)
+        q = reinterpret_cast<double2 *>(z.p[2]) + (i - a - b);
+    if (q) *q = make_double2(0.0, 0.0);
+    if (i == 0)
+        for (int k = 0; k < 3; ++k)
+            if (z.n[k] & 1) z.p[k][z.n[k] - 1] = 0.0;
+}
+
 __global__ void k_zero_unaligned(double *p, int64_t n)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -90,6 +116,12 @@ hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64
 {
     const int64_t len[3] = {n, 3 * n, n};
     double *ptr[3] = {Rmass, Rmom, Rener};
+    if ((((uintptr_t)Rmass | (uintptr_t)Rmom | (uintptr_t)Rener) & 15u) == 0 && 5 * n < (int64_t(1) << 24)) {
+        const ZeroSet z{{Rmass, Rmom, Rener}, {len[0], len[1], len[2]}};
+        const int64_t blocks = std::min<int64_t>((len[0] / 2 + len[1] / 2 + len[2] / 2) / 256 + 1, 0x7fffffff);
+        hipLaunchKernelGGL(k_zero3, dim3((unsigned)blocks), dim3(256), 0, s, z);
+        return hipGetLastError();
+    }
     for (int k = 0; k < 3; ++k) {
         const bool aligned = ((uintptr_t)ptr[k] & 15u) == 0;
         // 16-byte stores need a 16-byte aligned base; otherwise one double at a time
