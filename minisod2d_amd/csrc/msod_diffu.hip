@@ -13,8 +13,8 @@
 //        F_d = gpvol * sum_i He(i,d) Phi_i        (6 values per node)
 //    and the target accumulates the transposed sweep
 //        W[Phi](a) += sum_n D[n][I_d(a)] F_d(s_d(n))
-//    (LDS holds 6 values per node instead of 18, in the buffer pass 1 used
-//    for rho, T, u).  Then
+//    (LDS holds 6 values per node instead of 18, node-major, in the buffer
+//    pass 1 used for rho, T, u).  Then
 //    Rmass += nu_e W[grad rho], Rener += W[tau.u] + kappa_e W[grad T],
 //    Rmom_r += W[tau_r], with nu_e and kappa_e of the target node (the
 //    reference multiplies kappa_e inside the sum; same value, other rounding).
@@ -103,12 +103,12 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
     using S = Shape<P>;
     using T = TA;  // arithmetic: FP64, or FP32 for the FP32-arithmetic form (node sums stay FP64)
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
-    // sA (pass 1: rho, T, u) and sG (pass 2: fluxes of one direction) share
-    // one buffer; so does the MLP scratch of the closure between the two
-    constexpr size_t kSA = sizeof(T) * EPB * 5 * NN, kSG = sizeof(T) * EPB * 6 * NN;
+    // one buffer of node records (sN) serves pass 1 (rho, T, u) and pass 2
+    // (fluxes of one direction); the MLP scratch of the closure overlays it
+    // between the two
+    constexpr size_t kRec = sizeof(T) * EPB * 6 * NN;
     constexpr size_t kScr = CLOSURE ? (size_t)(S::NT / 64) * 64 * kMlpH * sizeof(short) : 0;
-    constexpr size_t kAG = kSA > kSG ? kSA : kSG;
-    constexpr size_t kBuf = kAG > kScr ? kAG : kScr;
+    constexpr size_t kBuf = kRec > kScr ? kRec : kScr;
     __shared__ T sD[Q * Q];
     __shared__ __attribute__((aligned(16))) unsigned char sbuf[kBuf];
     __shared__ double sR[5][UMAX];
@@ -122,8 +122,12 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
     constexpr int ME = kMuPerPass ? EPB : S::BATCH;
     __shared__ TG sMu[2][ME][NN];
     __shared__ typename std::conditional<CLOSURE, MlpShared, char>::type smlp;
-    auto sA = reinterpret_cast<T (*)[5][NN]>(sbuf);  // rho, T, u0, u1, u2
-    auto sG = reinterpret_cast<T (*)[6][NN]>(sbuf);  // contravariant fluxes of one direction
+    // node-major records of 6 values (pass 1: rho, T, u0, u1, u2, unused;
+    // pass 2: the contravariant fluxes of one direction), read in pairs: in
+    // FP64 three ds_read_b128 per source instead of ds_read2_b64 pairs
+    // (diffusion 13.71 -> 13.51 ms at 128^3 p=4, 3.81 -> 3.67 ms at 48^3 p=7)
+    struct alignas(2 * sizeof(T)) V2 { T x, y; };
+    auto sN = reinterpret_cast<V2 (*)[NN][3]>(sbuf);
 
     const Plan &pl = a.plan;
     constexpr int NT = S::NT, UPT = (UMAX + NT - 1) / NT;
@@ -196,18 +200,16 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
         T r = 0, muf = 0, mue = 0, mus = 0, uu[3] = {0, 0, 0};
         // the previous pass read its sMu entries before its pass-2 barriers
         if (kMuPerPass) diffu_stage_mu<NN, ME, NT, CLOSURE>(a, b * S::BATCH + pass * EPB, sMu);
-        __syncthreads();  // previous pass done with sA/sG (and sR zeroed, sMu staged)
+        __syncthreads();  // previous pass done with sN (and sR zeroed, sMu staged)
         if (active) {
             r = cur.r;
             muf = cur.muf;
             mue = (T)sMu[0][kMuPerPass ? le : pass * EPB + le][l];
             mus = (T)sMu[1][kMuPerPass ? le : pass * EPB + le][l];
-            sA[le][0][l] = r;
-            sA[le][1][l] = cur.T_;
-            for (int d = 0; d < 3; ++d) {
-                uu[d] = cur.u[d];
-                sA[le][2 + d][l] = uu[d];
-            }
+            for (int d = 0; d < 3; ++d) uu[d] = cur.u[d];
+            sN[le][l][0] = V2{r, (T)cur.T_};
+            sN[le][l][1] = V2{uu[0], uu[1]};
+            sN[le][l][2] = V2{uu[2], (T)0};
         }
         if (pass + 1 < S::NPASS) {  // next pass's inputs: in flight during this pass
             const int64_t en = e + EPB;
@@ -235,11 +237,12 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
                 for (int n = 0; n < Q; ++n) {
                     const int s = base + n * stride;
                     const T Dv = sD[Id * Q + n];
-                    aR += Dv * sA[le][0][s];
-                    aT += Dv * sA[le][1][s];
-                    aU0 += Dv * sA[le][2][s];
-                    aU1 += Dv * sA[le][3][s];
-                    aU2 += Dv * sA[le][4][s];
+                    const V2 p0 = sN[le][s][0], p1 = sN[le][s][1], p2 = sN[le][s][2];
+                    aR += Dv * p0.x;
+                    aT += Dv * p0.y;
+                    aU0 += Dv * p1.x;
+                    aU1 += Dv * p1.y;
+                    aU2 += Dv * p2.x;
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
@@ -253,7 +256,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
             }
         }
         if constexpr (CLOSURE) {
-            __syncthreads();  // every sweep over sA done: the MLP scratch overlays it
+            __syncthreads();  // every pass-1 sweep over sN done: the MLP scratch overlays it
             float f[kMlpIn];
             if (active)
                 mlp_features<T>(G, r, uu, h, f);
@@ -283,18 +286,15 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
         }
 
         // ---- pass 2: transposed sweeps, one reference direction at a time -
-        __syncthreads();  // every sweep over sA done: sG overlays it
+        __syncthreads();  // every pass-1 sweep over sN done: the fluxes overwrite it
         T W[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (active) {
                 const T c0 = gv * h[0][d], c1 = gv * h[1][d], c2 = gv * h[2][d];
-                sG[le][0][l] = c0 * gR[0] + c1 * gR[1] + c2 * gR[2];
-                sG[le][1][l] = c0 * tauU[0] + c1 * tauU[1] + c2 * tauU[2];
-                sG[le][2][l] = c0 * gT[0] + c1 * gT[1] + c2 * gT[2];
-                sG[le][3][l] = c0 * tau[0][0] + c1 * tau[0][1] + c2 * tau[0][2];
-                sG[le][4][l] = c0 * tau[1][0] + c1 * tau[1][1] + c2 * tau[1][2];
-                sG[le][5][l] = c0 * tau[2][0] + c1 * tau[2][1] + c2 * tau[2][2];
+                sN[le][l][0] = V2{c0 * gR[0] + c1 * gR[1] + c2 * gR[2], c0 * tauU[0] + c1 * tauU[1] + c2 * tauU[2]};
+                sN[le][l][1] = V2{c0 * gT[0] + c1 * gT[1] + c2 * gT[2], c0 * tau[0][0] + c1 * tau[0][1] + c2 * tau[0][2]};
+                sN[le][l][2] = V2{c0 * tau[1][0] + c1 * tau[1][1] + c2 * tau[1][2], c0 * tau[2][0] + c1 * tau[2][1] + c2 * tau[2][2]};
             }
             __syncthreads();
             if (active) {
@@ -306,10 +306,14 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
                     const int s = base + n * stride;
                     const T Dv = sD[n * Q + Id];  // transposed: l_{I_d(a)}'(xi_n)
 #pragma unroll
-                    for (int f = 0; f < 6; ++f) W[f] += Dv * sG[le][f][s];
+                    for (int f = 0; f < 3; ++f) {
+                        const V2 pf = sN[le][s][f];
+                        W[2 * f] += Dv * pf.x;
+                        W[2 * f + 1] += Dv * pf.y;
+                    }
                 }
             }
-            __syncthreads();  // sweep done before sG is overwritten
+            __syncthreads();  // sweep done before sN is overwritten
         }
         double v[5] = {0, 0, 0, 0, 0};
         if (active) {
