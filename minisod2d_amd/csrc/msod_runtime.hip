@@ -580,6 +580,21 @@ void drop_derived_in(uintptr_t lo, uintptr_t hi)
         }
 }
 
+// The contents of an array changed (the stream is synchronised): a present
+// array takes a new version, and every layout derived from its device copy
+// is dropped, also those cached for calls that passed the device address
+// itself (version 0), which a new version alone would leave in use.
+void contents_changed(const void *ptr)
+{
+    uintptr_t base;
+    if (Region *r = find_region(ptr, &base)) {
+        r->version = g.next_version++;
+        drop_derived_in((uintptr_t)r->dev, (uintptr_t)r->dev + r->bytes);
+    } else {
+        drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+    }
+}
+
 // --- timing ---------------------------------------------------------------
 hipEvent_t take_event()
 {
@@ -1058,7 +1073,7 @@ int msod_data_update_device(const void *host, size_t bytes)
     if (off + bytes > r->bytes) return fail(MSOD_ERR_ARG, "update_device: range exceeds the array");
     HIPCHK(hipStreamSynchronize(g.stream), "update_device");
     HIPCHK(hipMemcpy((char *)r->dev + off, host, bytes, hipMemcpyHostToDevice), "update_device");
-    r->version = g.next_version++;
+    contents_changed(host);
     return MSOD_OK;
 }
 
@@ -1109,13 +1124,8 @@ int msod_data_invalidate(const void *ptr)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     NEED_INIT();
-    uintptr_t base;
-    if (Region *r = find_region(ptr, &base)) {
-        r->version = g.next_version++;
-        return MSOD_OK;
-    }
     HIPCHK(hipStreamSynchronize(g.stream), "invalidate");
-    drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+    contents_changed(ptr);
     return MSOD_OK;
 }
 
@@ -1265,11 +1275,7 @@ int msod_elem_jacobian(int porder, int nelem, int npoin, const int32_t *connec, 
     HIPCHK(hipStreamSynchronize(g.stream), "geometry kernel");
     // layouts derived from He and gpvol are stale now
     for (const void *ptr : {(const void *)He, (const void *)gpvol}) {
-        uintptr_t base;
-        if (Region *r = find_region(ptr, &base))
-            r->version = g.next_version++;
-        else
-            drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+        contents_changed(ptr);
     }
     if (counts[0])
         return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%d)",
@@ -1346,11 +1352,7 @@ int msod_box_mesh(int nx, int ny, int nz, int porder, double jitter, int64_t see
     tk.done();
     HIPCHK(hipStreamSynchronize(g.stream), "box mesh");  // the table is freed on return
     for (const void *ptr : {(const void *)connec, (const void *)coord}) {
-        uintptr_t base;
-        if (Region *r = find_region(ptr, &base))
-            r->version = g.next_version++;
-        else
-            drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+        contents_changed(ptr);
     }
     return finish_call("box mesh");
 }
@@ -1553,11 +1555,7 @@ int msod_tet_jacobian(int nelem, int npoin, const int32_t *connec, const double 
            "TET geometry counters");
     HIPCHK(hipStreamSynchronize(g.stream), "TET geometry kernel");
     for (const void *ptr : {(const void *)He, (const void *)gpvol}) {
-        uintptr_t base;
-        if (Region *r = find_region(ptr, &base))
-            r->version = g.next_version++;
-        else
-            drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
+        contents_changed(ptr);
     }
     if (counts[0])
         return fail(MSOD_ERR_MESH, "connec holds %llu node numbers outside 1..npoin (npoin=%d)",

@@ -118,6 +118,70 @@ def test_shared_node_mesh(gpu):
         assert np.all(np.asarray(got)[npoin - 50:] == 0.0), key
 
 
+def _tet_on(rng, connec, npoin, coord, fields):
+    """TetProblem of a given connectivity: He, gpvol from the host routine."""
+    nelem = connec.shape[0]
+    t = m.tet_tables()
+    He = np.zeros((3, 3, 4, nelem), order="F")
+    gp = np.zeros((1, 4, nelem), order="F")
+    m.host_lib().msodh_elem_jacobian(nelem, npoin, 4, connec.ctypes.data, coord.ctypes.data,
+                                     t.dNgp.ctypes.data, t.wgp.ctypes.data, gp.ctypes.data,
+                                     He.ctypes.data)
+    return m.TetProblem(t, nelem, npoin, connec, coord, He, gp, *fields)
+
+
+def test_connectivity_rewritten_in_place(gpu):
+    """The mesh analysis is cached per connectivity (plain stores when no
+    node is shared).  A disjoint connectivity rewritten in place into one
+    with shared nodes, then msod_data_update_device: calls through the host
+    address and through the device address (msod_data_deviceptr) both see
+    the new mesh and match the oracle."""
+    say("[gpu] TET connectivity rewritten in place")
+    rng = np.random.default_rng(23)
+    nelem, npoin = 500, 2000
+    coord = np.asfortranarray(rng.random((npoin, 3)))
+    f = lambda *sh: np.asfortranarray(rng.random(sh))
+    fields = (f(npoin, 3), f(npoin, 3), 1 + f(npoin), 1 + f(npoin), 2 + f(npoin), 1 + f(npoin),
+              f(npoin), f(nelem, 4), f(nelem, 4))
+    disjoint = np.asfortranarray(np.arange(1, 4 * nelem + 1, dtype=np.int32).reshape(nelem, 4))
+    shared = np.asfortranarray(np.array([rng.choice(npoin - 50, 4, replace=False) + 1
+                                         for _ in range(nelem)], dtype=np.int32))
+    A = _tet_on(rng, disjoint, npoin, coord, fields)
+    B = _tet_on(rng, shared, npoin, coord, fields)
+    # one set of host arrays, made present once; A's contents first
+    tp = _tet_on(rng, disjoint.copy(order="F"), npoin, coord, fields)
+    R, D = tp.outputs(), tp.outputs()
+    ins = tp.device_inputs()
+    m.copyin(*ins)
+    m.create(*R, *D)
+    try:
+        dc = m.hip_lib().msod_data_deviceptr(tp.connec.ctypes.data)
+        assert dc
+        for want_p, label in ((A, "disjoint"), (B, "shared")):
+            if want_p is B:
+                tp.connec[...] = B.connec
+                tp.He[...] = B.He
+                tp.gpvol[...] = B.gpvol
+                m.update_device(tp.connec, tp.He, tp.gpvol)
+            want = oracle_tet(want_p)
+            for via, conn in (("host address", tp.connec), ("device address", dc)):
+                args_c = (nelem, npoin, conn) + tp.convec_args(*R)[3:]
+                args_d = (nelem, npoin, conn) + tp.diffu_args(*D)[3:]
+                m.fem_convec(*args_c)
+                m.fem_diffu(*args_d)
+                m.update_host(*R, *D)
+                for key, got, w in zip(KEYS, R + D, want):
+                    assert rel_l2(got, w) <= TOL, (label, via, key, rel_l2(got, w))
+        # invalidating the device address keeps the calls right
+        m._check(m.hip_lib().msod_data_invalidate(dc))
+        m.fem_convec(*((nelem, npoin, dc) + tp.convec_args(*R)[3:]))
+        m.update_host(*R)
+        for key, got, w in zip(KEYS, R, oracle_tet(B)[:3]):
+            assert rel_l2(got, w) <= TOL, ("after invalidate", key, rel_l2(got, w))
+    finally:
+        m.delete(*ins, *R, *D)
+
+
 def test_tet_jacobian_matches_golden_and_host(gpu):
     from golden.make_golden import digest
     say("[gpu] TET geometry vs the reference's tet_jacobian")
