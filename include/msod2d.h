@@ -67,7 +67,22 @@ int msod_data_update_host(void *host, size_t bytes);
 int msod_data_delete(const void *host);
 int msod_data_present(const void *host);         /* 1 if present, else 0 */
 void *msod_data_deviceptr(const void *host);     /* NULL if not present */
-/* Forget layouts derived from a device array changed outside msod_data_* */
+/* Layouts derived from an array (the assembly plan from connec, the
+ * element-major He and gpvol, the TET04 mesh analysis) are cached per array
+ * version.  Arrays of the data region change version with
+ * msod_data_update_device, msod_data_invalidate and the library's own device
+ * routines that write them; the device address of a present array
+ * (msod_data_deviceptr) shares its version.  Device memory outside the data
+ * region has no version: its derived layouts are rebuilt on every call (the
+ * reference re-reads every array per call, elem_convec.f90:228).
+ *
+ * msod_data_register enters a device array the caller owns (hipMalloc, a
+ * torch tensor) into the data region, keyed by its device address, so that
+ * its derived layouts are cached; the caller then calls msod_data_invalidate
+ * after changing its contents, and msod_data_delete to remove it (the
+ * library never frees it).  No counterpart in the reference. */
+int msod_data_register(const void *dev, size_t bytes);
+/* Forget layouts derived from an array changed outside msod_data_* */
 int msod_data_invalidate(const void *ptr);
 
 /* ---- execution control -------------------------------------------------- */

@@ -58,7 +58,7 @@ ABI_SYMBOLS = (
     "msod_closure_sensitivity", "msod_closure_gradient", "msod_closure_adam",
     "msod_closure_loss", "msod_elem_jacobian", "msod_box_mesh", "msod_flow_fields",
     "msod_viscosity_fields", "msod_tet_jacobian", "msod_fem_convec", "msod_fem_diffu",
-    "msod_full_convec_ijk_fp32", "msod_full_diffusion_ijk_fp32",
+    "msod_full_convec_ijk_fp32", "msod_full_diffusion_ijk_fp32", "msod_data_register",
 )
 
 
@@ -95,7 +95,7 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_data_deviceptr.restype = _vp
         L.msod_data_deviceptr.argtypes = [_vp]
         for n in ("msod_data_copyin", "msod_data_create", "msod_data_update_device",
-                  "msod_data_update_host"):
+                  "msod_data_update_host", "msod_data_register"):
             getattr(L, n).argtypes = [_vp, ctypes.c_size_t]
         for n in ("msod_data_delete", "msod_data_present", "msod_data_invalidate"):
             getattr(L, n).argtypes = [_vp]

@@ -5,7 +5,10 @@
 //   * the sorted distinct nodes of every batch (unode, offsets uoff) and the
 //     position of every element node in its batch's list (lidx);
 //   * a greedy colouring of the batches such that no two batches of one
-//     colour share a node (at most 32 colours);
+//     colour share a node (at most 32 colours), used only by the coloured
+//     assembly; a mesh whose batches need more colours (elements in no
+//     spatial order) gets no colouring and runs with the atomic assembly
+//     alone (colored = false);
 //   * bit 31 of unode marks the first writer of a node: the batch of the
 //     lowest colour holding it;
 //   * the batches grouped by colour (order, color_off) and the nodes no
@@ -32,6 +35,7 @@ struct HostPlan {
     std::vector<uint32_t> order;      // [nbatch]
     std::vector<int64_t> color_off;   // [ncolors+1]
     std::vector<uint32_t> unref;
+    bool colored = false;             // colouring built (order, color_off, bit 31 valid)
 };
 
 template <class F>
@@ -131,21 +135,38 @@ inline bool build_host_plan(const int32_t *raw, int64_t nelem, int64_t npoin,
     }
 
     // greedy colouring: bit c of mask[node] is set once a batch of colour c
-    // holds the node
+    // holds the node.  Past 32 colours the colouring is abandoned: the
+    // atomic assembly does not need it.
     std::vector<uint32_t> mask(npoin, 0u);
     std::vector<uint8_t> color(NB);
     int ncolors = 0;
-    for (int64_t b = 0; b < NB; ++b) {
+    bool overflow = false;
+    for (int64_t b = 0; b < NB && !overflow; ++b) {
         uint32_t forb = 0;
         for (uint32_t k = P.uoff[b]; k < P.uoff[b + 1]; ++k) forb |= mask[P.unode[k]];
         if (forb == 0xffffffffu) {
-            *err = "mesh needs more than 32 batch colours";
-            return false;
+            overflow = true;
+            break;
         }
         const int c = __builtin_ctz(~forb);
         color[b] = (uint8_t)c;
         ncolors = std::max(ncolors, c + 1);
         for (uint32_t k = P.uoff[b]; k < P.uoff[b + 1]; ++k) mask[P.unode[k]] |= 1u << c;
+    }
+    if (overflow) {
+        // atomic assembly only: no colours, no first-writer flags; the
+        // unreferenced nodes from a plain reference count
+        std::vector<uint8_t> ref(npoin, 0);
+        for (uint32_t v : P.unode) ref[v] = 1;
+        P.unref.clear();
+        for (int64_t i = 0; i < npoin; ++i)
+            if (!ref[i]) P.unref.push_back((uint32_t)i);
+        P.color_off.assign(2, 0);
+        P.color_off[1] = NB;
+        P.order.resize(NB);
+        for (int64_t b = 0; b < NB; ++b) P.order[b] = (uint32_t)b;
+        P.colored = false;
+        return true;
     }
     plan_parallel_for(NB, [&](int64_t lo, int64_t hi) {
         for (int64_t b = lo; b < hi; ++b) {
@@ -163,6 +184,7 @@ inline bool build_host_plan(const int32_t *raw, int64_t nelem, int64_t npoin,
     P.order.assign(NB, 0u);
     std::vector<int64_t> pos(P.color_off.begin(), P.color_off.end() - 1);
     for (int64_t b = 0; b < NB; ++b) P.order[pos[color[b]]++] = (uint32_t)b;
+    P.colored = true;
     return true;
 }
 

@@ -6,6 +6,8 @@
 #include "msod_closure.h"
 #include "msod_gen.h"
 #include "msod_geom.h"
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "msod_kernels.h"
 #include "msod_mlp.h"
 #include "msod_plan.h"
@@ -23,6 +25,19 @@
 #include <type_traits>
 #include <atomic>
 #include <vector>
+
+// roctx range around each entry point: the counterpart of the reference's
+// nvtxStartRange/nvtxEndRange inside both kernels
+// (src/utils/sources/mod_nvtx.f90:54-80, elem_convec.f90:221,429,
+// elem_diffu.f90:34,196), without the device synchronisation those force
+// (mod_nvtx.f90:60,77), so the ranges do not change the timing.  Visible in
+// rocprofv3 --marker-trace.
+struct MsodRange {
+    explicit MsodRange(const char *name) { roctxRangePushA(name); }
+    ~MsodRange() { roctxRangePop(); }
+    MsodRange(const MsodRange &) = delete;
+    MsodRange &operator=(const MsodRange &) = delete;
+};
 
 namespace msod {
 
@@ -193,10 +208,15 @@ namespace {
 
 using namespace msod;
 
+// An array of the data region.  Keyed by its host address (msod_data_create,
+// msod_data_copyin) or, for a device array the caller owns and registered
+// with msod_data_register, by its device address (owned = false: never
+// freed by the library).
 struct Region {
     size_t bytes;
     void *dev;
     uint64_t version;
+    bool owned = true;
 };
 
 struct TableSet {
@@ -222,6 +242,7 @@ struct PlanEntry {
     std::vector<int64_t> color_off;  // batches of colour c: order[color_off[c] .. color_off[c+1])
     uint32_t *unref = nullptr;       // nodes no element references (zeroed per call)
     int64_t nunref = 0;
+    bool colored = false;            // batch colouring built (the coloured assembly needs it)
     void release()
     {
         for (void *p : {(void *)conn, (void *)uoff, (void *)unode, (void *)order, (void *)lidx,
@@ -291,6 +312,7 @@ struct State {
 
 State g;
 std::recursive_mutex g_mu;
+
 thread_local std::string g_err;
 
 int fail(int status, const char *fmt, ...)
@@ -334,6 +356,17 @@ Region *find_region(const void *p, uintptr_t *base)
     return nullptr;
 }
 
+// Region whose DEVICE copy holds address p (a present array reached through
+// msod_data_deviceptr, or a registered device array).
+Region *find_region_dev(const void *p)
+{
+    for (auto &kv : g.present) {
+        const uintptr_t lo = (uintptr_t)kv.second.dev;
+        if ((uintptr_t)p >= lo && (uintptr_t)p < lo + kv.second.bytes) return &kv.second;
+    }
+    return nullptr;
+}
+
 bool is_device_pointer(const void *p)
 {
     hipPointerAttribute_t attr;
@@ -359,7 +392,19 @@ int resolve(const void *p, size_t need, const char *name, const void **dev, uint
         *ver = r->version;
         return MSOD_OK;
     }
+    if (Region *r = find_region_dev(p)) {  // the device address of a present array
+        const size_t off = (uintptr_t)p - (uintptr_t)r->dev;
+        if (off + need > r->bytes)
+            return fail(MSOD_ERR_MESH, "%s: present array holds %zu bytes past this address, %zu needed",
+                        name, r->bytes - off, need);
+        *dev = p;
+        *ver = r->version;
+        return MSOD_OK;
+    }
     if (is_device_pointer(p)) {
+        // device memory outside the data region: version 0, and every layout
+        // derived from it is rebuilt on each call (the caller may change it
+        // without telling the library; msod_data_register caches them)
         *dev = p;
         *ver = 0;
         return MSOD_OK;
@@ -479,6 +524,7 @@ int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet
     P.B = H.B;
     P.nbatch = H.nbatch;
     P.color_off = H.color_off;
+    P.colored = H.colored;
     int st;
     if ((st = upload(H.conn, &P.conn, "plan upload"))) return st;
     if ((st = upload(H.lidx, &P.lidx, "plan upload"))) return st;
@@ -495,9 +541,33 @@ int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet
     return MSOD_OK;
 }
 
+// Layouts derived from device memory outside the data region (version 0) are
+// not reused from call to call: the array may have changed since.
+void drop_unversioned(const void *src)
+{
+    for (size_t i = g.plans.size(); i-- > 0;)
+        if (g.plans[i].src == src && g.plans[i].ver == 0) {
+            g.plans[i].release();
+            g.plans.erase(g.plans.begin() + i);
+        }
+    for (size_t i = g.checked.size(); i-- > 0;)
+        if (g.checked[i].src == src && g.checked[i].ver == 0) {
+            g.checked[i].release();
+            g.checked.erase(g.checked.begin() + i);
+        }
+    for (size_t i = g.geos.size(); i-- > 0;)
+        if ((g.geos[i].srcHe == src && g.geos[i].verHe == 0) ||
+            (g.geos[i].srcVol == src && g.geos[i].verVol == 0)) {
+            (void)hipFree(g.geos[i].geo);
+            (void)hipFree(g.geos[i].gvol);
+            g.geos.erase(g.geos.begin() + i);
+        }
+}
+
 int get_plan(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, const TableSet *ts,
              const PlanEntry **out)
 {
+    if (ver == 0) drop_unversioned(dconnec);
     for (auto &c : g.plans)
         if (c.src == dconnec && c.ver == ver && c.nelem == nelem && c.npoin == npoin &&
             c.porder == ts->porder && c.l2g == ts->l2g) {
@@ -529,6 +599,8 @@ template <typename T>
 int get_geo(const void *dHe, uint64_t vHe, const void *dVol, uint64_t vVol, int64_t nelem,
             const TableSet *ts, const T **geo, const T **gvol)
 {
+    if (vHe == 0) drop_unversioned(dHe);
+    if (vVol == 0) drop_unversioned(dVol);
     for (auto &c : g.geos)
         if (c.srcHe == dHe && c.srcVol == dVol && c.verHe == vHe && c.verVol == vVol &&
             c.nelem == nelem && c.porder == ts->porder && c.elt == sizeof(T) && c.l2g == ts->l2g) {
@@ -539,8 +611,12 @@ int get_geo(const void *dHe, uint64_t vHe, const void *dVol, uint64_t vVol, int6
     const int nn = (int)ts->l2g.size();
     DerivedGeo c{dHe, dVol, vHe, vVol, nelem, ts->porder, sizeof(T), ts->l2g, nullptr, nullptr};
     HIPCHK(hipMalloc(&c.geo, sizeof(T) * 9 * nelem * nn), "hipMalloc element-major He");
-    HIPCHK(hipMalloc(&c.gvol, sizeof(T) * nelem * nn), "hipMalloc element-major gpvol");
-    hipError_t e = launch_build_geo<T>((const T *)dHe, (const T *)dVol, nelem, nn, ts->dl2g,
+    hipError_t e = hipMalloc(&c.gvol, sizeof(T) * nelem * nn);
+    if (e != hipSuccess) {
+        (void)hipFree(c.geo);
+        return hipfail(e, "hipMalloc element-major gpvol");
+    }
+    e = launch_build_geo<T>((const T *)dHe, (const T *)dVol, nelem, nn, ts->dl2g,
                                        (T *)c.geo, (T *)c.gvol, g.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
     if (e != hipSuccess) {
@@ -587,12 +663,23 @@ void drop_derived_in(uintptr_t lo, uintptr_t hi)
 void contents_changed(const void *ptr)
 {
     uintptr_t base;
-    if (Region *r = find_region(ptr, &base)) {
+    Region *r = find_region(ptr, &base);
+    if (!r) r = find_region_dev(ptr);
+    if (r) {
         r->version = g.next_version++;
         drop_derived_in((uintptr_t)r->dev, (uintptr_t)r->dev + r->bytes);
     } else {
         drop_derived_in((uintptr_t)ptr, (uintptr_t)ptr + 1);
     }
+}
+
+// A device routine wrote the array at ptr (host key or device address).
+void bump_version(const void *ptr)
+{
+    uintptr_t base;
+    Region *r = find_region(ptr, &base);
+    if (!r) r = find_region_dev(ptr);
+    if (r) r->version = g.next_version++;
 }
 
 // --- timing ---------------------------------------------------------------
@@ -681,8 +768,17 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     pl.uoff = P.uoff;
     pl.unode = P.unode;
     pl.mode = g.assembly;
-    pl.kpb = kBatchesPerBlock;
+    // batches per convection workgroup: kBatchesPerBlock on large meshes,
+    // fewer where that would leave the launch short of 2048 workgroups (8 per
+    // CU); 8^3 p=3 has 64 batches, which kBatchesPerBlock put on 8 CUs
+    auto kpb_for = [](int64_t nb) {
+        return (int)std::max<int64_t>(1, std::min<int64_t>(kBatchesPerBlock, nb / 2048));
+    };
     const int64_t NP = a.npoin;
+    if (g.assembly == kColored && !P.colored)
+        return fail(MSOD_ERR_MESH, "%s: the batches of this mesh need more than 32 colours (elements in no "
+                                   "spatial order); the coloured assembly is not available, the atomic one "
+                                   "(MSOD_ASSEMBLY_ATOMIC, the default) handles any element order", what);
     {
         TimedLaunch tz(0);
         if (g.assembly == kAtomic)
@@ -696,12 +792,14 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     if (g.assembly == kAtomic) {
         pl.batches = natural ? P.natural : P.order;
         pl.nb = P.nbatch;
+        pl.kpb = kpb_for(pl.nb);
         a.plan = pl;
         HIPCHK(launch(a), what);
     } else {
         for (size_t c = 0; c + 1 < P.color_off.size(); ++c) {
             pl.batches = P.order + P.color_off[c];
             pl.nb = P.color_off[c + 1] - P.color_off[c];
+            pl.kpb = kpb_for(pl.nb);
             a.plan = pl;
             HIPCHK(launch(a), what);
         }
@@ -839,7 +937,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
 enum { kScrWeights = 0, kScrPartial = 1, kScrLoss = 2 };
 
 // Library-owned device buffer `slot` of at least `bytes` (grow-only).
-int scratch(int slot, size_t bytes, void **p)
+int library_buffer(int slot, size_t bytes, void **p)
 {
     if (g.scr_bytes[slot] < bytes) {
         HIPCHK(hipStreamSynchronize(g.stream), "closure buffer resize");
@@ -875,7 +973,7 @@ int closure_weights(const float *params, const float **dev, bool *usable)
     } else {
         std::memcpy(h.data(), params, bytes);
         void *buf;
-        int st = scratch(kScrWeights, bytes, &buf);
+        int st = library_buffer(kScrWeights, bytes, &buf);
         if (st) return st;
         HIPCHK(hipMemcpyAsync(buf, h.data(), bytes, hipMemcpyHostToDevice, g.stream), "uploading mlp_params");
         HIPCHK(hipStreamSynchronize(g.stream), "uploading mlp_params");
@@ -987,7 +1085,8 @@ int msod_finalize(void)
     if (!g.init) return MSOD_OK;
     (void)hipStreamSynchronize(g.stream);
     collect_timing();
-    for (auto &kv : g.present) (void)hipFree(kv.second.dev);
+    for (auto &kv : g.present)
+        if (kv.second.owned) (void)hipFree(kv.second.dev);
     g.present.clear();
     drop_derived_in(0, UINTPTR_MAX);
     for (TableSet *ts : g.tables) {
@@ -1069,6 +1168,9 @@ int msod_data_update_device(const void *host, size_t bytes)
     uintptr_t base;
     Region *r = host ? find_region(host, &base) : nullptr;
     if (!r) return fail(MSOD_ERR_NOT_PRESENT, "update_device: %p not present", host);
+    if (!r->owned)
+        return fail(MSOD_ERR_ARG, "update_device: %p is a registered device array (no host copy); "
+                                  "call msod_data_invalidate after changing it", host);
     const size_t off = (uintptr_t)host - base;
     if (off + bytes > r->bytes) return fail(MSOD_ERR_ARG, "update_device: range exceeds the array");
     HIPCHK(hipStreamSynchronize(g.stream), "update_device");
@@ -1084,6 +1186,8 @@ int msod_data_update_host(void *host, size_t bytes)
     uintptr_t base;
     Region *r = host ? find_region(host, &base) : nullptr;
     if (!r) return fail(MSOD_ERR_NOT_PRESENT, "update_host: %p not present", host);
+    if (!r->owned)
+        return fail(MSOD_ERR_ARG, "update_host: %p is a registered device array (no host copy)", host);
     const size_t off = (uintptr_t)host - base;
     if (off + bytes > r->bytes) return fail(MSOD_ERR_ARG, "update_host: range exceeds the array");
     HIPCHK(hipStreamSynchronize(g.stream), "update_host");
@@ -1100,8 +1204,28 @@ int msod_data_delete(const void *host)
     HIPCHK(hipStreamSynchronize(g.stream), "delete");
     const uintptr_t lo = (uintptr_t)it->second.dev, hi = lo + it->second.bytes;
     drop_derived_in(lo, hi);
-    HIPCHK(hipFree(it->second.dev), "hipFree");
+    if (it->second.owned) HIPCHK(hipFree(it->second.dev), "hipFree");
     g.present.erase(it);
+    return MSOD_OK;
+}
+
+int msod_data_register(const void *dev, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    NEED_INIT();
+    if (!dev || bytes == 0) return fail(MSOD_ERR_ARG, "data_register: NULL or empty");
+    uintptr_t base;
+    if (Region *r = find_region(dev, &base)) {
+        if (base == (uintptr_t)dev && r->bytes == bytes && !r->owned) return MSOD_OK;  // already registered
+        return fail(MSOD_ERR_ARG, "data_register: %p overlaps an array of the data region", dev);
+    }
+    if (find_region_dev(dev))
+        return fail(MSOD_ERR_ARG, "data_register: %p is the device copy of a present array", dev);
+    auto it = g.present.lower_bound((uintptr_t)dev);
+    if (it != g.present.end() && it->first < (uintptr_t)dev + bytes)
+        return fail(MSOD_ERR_ARG, "data_register: %p+%zu overlaps an array of the data region", dev, bytes);
+    if (!is_device_pointer(dev)) return fail(MSOD_ERR_ARG, "data_register: %p is not device memory", dev);
+    g.present[(uintptr_t)dev] = Region{bytes, const_cast<void *>(dev), g.next_version++, false};
     return MSOD_OK;
 }
 
@@ -1205,6 +1329,7 @@ int msod_elem_jacobian(int porder, int nelem, int npoin, const int32_t *connec, 
                        const double *dNgp, const double *wgp, double *gpvol, double *He, int64_t *nbad)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_elem_jacobian");
     NEED_INIT();
     int st = check_sizes(porder, nelem, npoin);
     if (st) return st;
@@ -1289,6 +1414,7 @@ int msod_box_mesh(int nx, int ny, int nz, int porder, double jitter, int64_t see
                   double *coord)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_box_mesh");
     NEED_INIT();
     if (porder < 1 || porder > kGenMaxP)
         return fail(MSOD_ERR_PORDER, "porder=%d: the generator supports 1..%d", porder, kGenMaxP);
@@ -1362,6 +1488,7 @@ int msod_flow_fields(int64_t npoin, const double *coord, double gamma, double Rg
                      double *Tem, double *mu_fluid)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_flow_fields");
     NEED_INIT();
     if (npoin < 1 || !(gamma > 1.0) || !(Rgas > 0.0))
         return fail(MSOD_ERR_ARG, "npoin=%lld gamma=%g Rgas=%g", (long long)npoin, gamma, Rgas);
@@ -1393,7 +1520,8 @@ int msod_flow_fields(int64_t npoin, const double *coord, double gamma, double Rg
     tk.done();
     for (int k = 1; k < 8; ++k) {
         uintptr_t base;
-        if (Region *r = find_region(in[k], &base)) r->version = g.next_version++;
+        (void)base;
+        bump_version(in[k]);
     }
     return finish_call("flow fields");
 }
@@ -1402,6 +1530,7 @@ int msod_viscosity_fields(int64_t nelem, int ngaus, int64_t seed_mue, int64_t se
                           double musgs_max, double *mu_e, double *mu_sgs)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_viscosity_fields");
     NEED_INIT();
     if (nelem < 1 || ngaus < 1) return fail(MSOD_ERR_ARG, "nelem=%lld ngaus=%d", (long long)nelem, ngaus);
     ViscArgs v{};
@@ -1423,7 +1552,8 @@ int msod_viscosity_fields(int64_t nelem, int ngaus, int64_t seed_mue, int64_t se
     tk.done();
     for (const void *ptr : {(const void *)mu_e, (const void *)mu_sgs}) {
         uintptr_t base;
-        if (Region *r = find_region(ptr, &base)) r->version = g.next_version++;
+        (void)base;
+        bump_version(ptr);
     }
     return finish_call("viscosity fields");
 }
@@ -1459,6 +1589,7 @@ static int check_tet_connec(const void *dc, uint64_t ver, int64_t nelem, int64_t
                             const CheckedMesh **out)
 {
     const int64_t n = 4 * nelem;
+    if (ver == 0) drop_unversioned(dc);
     for (const CheckedMesh &c : g.checked)
         if (c.src == dc && c.ver == ver && c.n == n && c.npoin == npoin) {
             *out = &c;
@@ -1524,6 +1655,7 @@ int msod_tet_jacobian(int nelem, int npoin, const int32_t *connec, const double 
                       const double *dNgp, const double *wgp, double *gpvol, double *He, int64_t *nbad)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_tet_jacobian");
     NEED_INIT();
     int st = check_tet_sizes(nelem, npoin);
     if (st) return st;
@@ -1570,6 +1702,7 @@ int msod_fem_convec(int nelem, int npoin, const int32_t *connec, const double *N
                     double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_fem_convec");
     NEED_INIT();
     int st = check_tet_sizes(nelem, npoin);
     if (st) return st;
@@ -1619,6 +1752,7 @@ int msod_fem_diffu(int nelem, int npoin, const int32_t *connec, const double *Ng
                    const double *mu_sgs, double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_fem_diffu");
     NEED_INIT();
     int st = check_tet_sizes(nelem, npoin);
     if (st) return st;
@@ -1686,6 +1820,7 @@ int msod_full_convec_ijk(int porder, int nelem, int npoin, const int32_t *connec
                          const double *E, double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_convec_ijk");
     (void)Ngp; (void)dNgp; (void)xgp;
     if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
     return convec_common<double, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
@@ -1701,6 +1836,7 @@ int msod_full_convec_ijk_mixed(int porder, int nelem, int npoin, const int32_t *
                                double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_convec_ijk_mixed");
     if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
     return convec_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
@@ -1717,6 +1853,7 @@ int msod_full_diffusion_ijk(int porder, int nelem, int npoin, const int32_t *con
                             double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk");
     (void)Ngp; (void)dNgp; (void)xgp;
     return diffu_common<double, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
@@ -1732,6 +1869,7 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
                                   double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk_mixed");
     return diffu_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                mu_sgs, Rmass, Rmom, Rener, 4);
@@ -1745,6 +1883,7 @@ int msod_full_convec_ijk_fp32(int porder, int nelem, int npoin, const int32_t *c
                               double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_convec_ijk_fp32");
     if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
     return convec_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                 gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
@@ -1760,6 +1899,7 @@ int msod_full_diffusion_ijk_fp32(int porder, int nelem, int npoin, const int32_t
                                  double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk_fp32");
     return diffu_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                mu_sgs, Rmass, Rmom, Rener, 15, false, nullptr, true);
@@ -1773,6 +1913,7 @@ int msod_full_convec_ijk_geo32(int porder, int nelem, int npoin, const int32_t *
                                const double *E, double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_convec_ijk_geo32");
     if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
     return convec_common<float, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                         gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, 0.0, Rmass,
@@ -1788,6 +1929,7 @@ int msod_full_diffusion_ijk_geo32(int porder, int nelem, int npoin, const int32_
                                   double *Rmass, double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk_geo32");
     return diffu_common<float, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                        gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e,
                                        mu_sgs, Rmass, Rmom, Rener, 8);
@@ -1805,6 +1947,7 @@ int msod_full_diffusion_ijk_closure(int porder, int nelem, int npoin, const int3
                                     double *Rmom, double *Rener)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk_closure");
     (void)Ngp; (void)dNgp; (void)xgp;
     NEED_INIT();
     if (used_model) *used_model = 0;
@@ -1826,6 +1969,7 @@ int msod_closure_features(int porder, int nelem, int npoin, const int32_t *conne
                           const double *u, float *features)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_features");
     ClosureElemArgs a;
     int st = closure_elem_common(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                  gmshAtoI, gmshAtoJ, gmshAtoK, rho, u, a);
@@ -1845,6 +1989,7 @@ int msod_closure_predict(int porder, int nelem, const int32_t *invAtoIJK, const 
                          const float *features, double *mu_sgs)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_predict");
     NEED_INIT();
     int st = check_sizes(porder, nelem, 1);
     if (st) return st;
@@ -1874,6 +2019,7 @@ int msod_closure_sensitivity(int porder, int nelem, int npoin, const int32_t *co
                              const double *rerr, float *gnu)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_sensitivity");
     ClosureElemArgs a;
     int st = closure_elem_common(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
                                  gmshAtoI, gmshAtoJ, gmshAtoK, rho, u, a);
@@ -1898,6 +2044,7 @@ int msod_closure_gradient(int64_t nsamples, const float *mlp_params, const float
                           const float *gnu, float *grad)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_gradient");
     NEED_INIT();
     if (nsamples <= 0) return fail(MSOD_ERR_ARG, "nsamples=%lld", (long long)nsamples);
     const float *dmlp;
@@ -1912,7 +2059,7 @@ int msod_closure_gradient(int64_t nsamples, const float *mlp_params, const float
     if ((st = resolve(grad, sizeof(float) * kMlpTrain, "grad", &dgr, &vx))) return st;
     const int nblocks = (int)std::min<int64_t>((nsamples + 63) / 64, 1024);
     void *part;
-    if ((st = scratch(kScrPartial, sizeof(float) * kMlpTrain * nblocks, &part))) return st;
+    if ((st = library_buffer(kScrPartial, sizeof(float) * kMlpTrain * nblocks, &part))) return st;
     TimedLaunch tk(6);
     HIPCHK(launch_mlp_grad(dmlp, (const float *)df, (const float *)dg, nsamples, (float *)part, nblocks,
                            (float *)dgr, g.stream),
@@ -1925,6 +2072,7 @@ int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, 
                       double lr, double beta1, double beta2, double eps)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_adam");
     NEED_INIT();
     if (step < 1 || !(lr > 0) || !(beta1 >= 0 && beta1 < 1) || !(beta2 >= 0 && beta2 < 1) || !(eps > 0))
         return fail(MSOD_ERR_ARG, "adam: step=%d lr=%g beta1=%g beta2=%g eps=%g", step, lr, beta1, beta2, eps);
@@ -1947,6 +2095,7 @@ int msod_closure_loss(int npoin, const double *Rmom, const double *Rener, const 
                       const double *weights, double *rerr, double *loss)
 {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_loss");
     NEED_INIT();
     if (npoin <= 0 || !weights || !loss) return fail(MSOD_ERR_ARG, "npoin=%d, weights and loss needed", npoin);
     const int64_t NP = npoin;
@@ -1959,7 +2108,7 @@ int msod_closure_loss(int npoin, const double *Rmom, const double *Rener, const 
     if ((st = resolve(rerr, sizeof(double) * 4 * NP, "rerr", &de, &vx))) return st;
     const int nblocks = (int)std::min<int64_t>((NP + 255) / 256, 1024);
     void *buf;
-    if ((st = scratch(kScrLoss, sizeof(double) * (4 + nblocks), &buf))) return st;
+    if ((st = library_buffer(kScrLoss, sizeof(double) * (4 + nblocks), &buf))) return st;
     double *dw = (double *)buf, *lpart = dw + 4;
     std::vector<double> part(nblocks);
     HIPCHK(hipMemcpyAsync(dw, weights, sizeof(double) * 4, hipMemcpyHostToDevice, g.stream), "loss weights");

@@ -89,8 +89,10 @@ contains
       end if
    end subroutine check_shapes
 
-   ! Linear tetrahedra (elem_convec.f90:839-948); the contributions meet at the
-   ! nodes through FP64 atomics, as in the reference.
+   ! Linear tetrahedra (elem_convec.f90:839-948).  On a mesh where no node is
+   ! shared between element nodes (the reference's disjoint TET04 mesh) the
+   ! contributions are stored; otherwise they meet at the nodes through FP64
+   ! atomics, like the reference's atomic updates.
    subroutine fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, Rmass, Rmom, Rener)
       integer(4), intent(in) :: nelem, npoin
       integer(4), intent(in), contiguous :: connec(:,:)
@@ -184,7 +186,9 @@ contains
       used_model = used /= 0
    end subroutine full_diffusion_ijk_closure
 
-   ! Linear tetrahedra (elem_diffu.f90:199-317).
+   ! Linear tetrahedra (elem_diffu.f90:199-317): stores on a mesh without
+   ! shared nodes (the reference adds without atomics there too,
+   ! elem_diffu.f90:306-312), FP64 atomics otherwise.
    subroutine fem_diffu(nelem, npoin, connec, Ngp, dNgp, He, gpvol, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, &
                         mu_sgs, Rmass, Rmom, Rener)
       integer(4), intent(in) :: nelem, npoin

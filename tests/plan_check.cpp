@@ -66,6 +66,26 @@ int main(int argc, char **argv)
             ++i;
         }
     }
+    if (!P.colored) {
+        // atomic assembly only: no first-writer flags, order is the mesh
+        // order, unref exactly the nodes no batch lists
+        std::vector<char> refd(npoin, 0), unref(npoin, 0);
+        for (uint32_t v : P.unode) {
+            if (v >> 31) return die("first-writer flag without colouring", v);
+            refd[v] = 1;
+        }
+        for (uint32_t v : P.unref) {
+            if (v >= npoin || unref[v]) return die("unref entry", v);
+            unref[v] = 1;
+        }
+        for (int64_t v = 0; v < npoin; ++v)
+            if ((bool)refd[v] == (bool)unref[v]) return die("unref set", v);
+        if ((int64_t)P.order.size() != NB) return die("order size");
+        for (int64_t b = 0; b < NB; ++b)
+            if (P.order[b] != (uint32_t)b) return die("order", b);
+        std::printf("OK ncolors=0 atomic-only nbatch=%lld\n", (long long)NB);
+        return 0;
+    }
     // colours: order is a permutation grouped by colour; one colour's
     // batches share no node; exactly one first writer per referenced node, of
     // the lowest colour holding it

@@ -71,6 +71,18 @@ def test_shuffled_elements_and_unreferenced_nodes(checker, tmp_path, p):
     assert "ncolors=" in out
 
 
+@pytest.mark.parametrize("n,p", [(24, 1), (16, 3)])
+def test_shuffled_meshes_past_32_colours_plan_atomic_only(checker, tmp_path, n, p):
+    """Elements in random order need more than 32 batch colours: the plan is
+    still built (the atomic assembly needs no colouring), without colours."""
+    prob = m.box_problem(n, p)
+    rng = np.random.default_rng(11)
+    connec = np.asfortranarray(prob.connec[rng.permutation(prob.nelem)])
+    l2g = prob.tables.invAtoIJK.ravel(order="F") - 1
+    out = run(checker, tmp_path, connec, prob.npoin, l2g, BATCH[p])
+    assert "atomic-only" in out, out
+
+
 def test_disjoint_mesh_is_one_colour(checker, tmp_path):
     prob = m.reference_problem(10, 3)
     l2g = prob.tables.invAtoIJK.ravel(order="F") - 1
