@@ -291,6 +291,19 @@ int msod_closure_sensitivity(int porder, int nelem, int npoin, const int32_t *co
 /* grad[MSOD_CLOSURE_NTRAIN] = sum_samples gnu * d mu_sgs / d params */
 int msod_closure_gradient(int64_t nsamples, const float *mlp_params, const float *features,
                           const float *gnu, float *grad);
+/* msod_closure_gradient that allocates no device memory.  No counterpart in
+ * the reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, a tight training loop, harnesses that forbid hipMalloc).  The
+ * per-workgroup partial sums go to a fixed library array of 1024 x 1505
+ * floats (6.2 MB per GPU, held until the process ends) instead of a library
+ * buffer grown on demand; the workgroups stride over the samples, so any
+ * nsamples fits.  mlp_params must be present or device memory (a host block
+ * would need a device copy).  Same kernels and results as
+ * msod_closure_gradient.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two library contexts on one device, each with its own
+ * stream) race. */
+int msod_closure_gradient_noalloc(int64_t nsamples, const float *mlp_params, const float *features,
+                                  const float *gnu, float *grad);
 /* one Adam step on the trained part of mlp_params (m, v: moment buffers) */
 int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, int step,
                       double lr, double beta1, double beta2, double eps);

@@ -59,6 +59,7 @@ ABI_SYMBOLS = (
     "msod_closure_loss", "msod_elem_jacobian", "msod_box_mesh", "msod_flow_fields",
     "msod_viscosity_fields", "msod_tet_jacobian", "msod_fem_convec", "msod_fem_diffu",
     "msod_full_convec_ijk_fp32", "msod_full_diffusion_ijk_fp32", "msod_data_register",
+    "msod_closure_gradient_noalloc",
 )
 
 
@@ -115,6 +116,7 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_closure_predict.argtypes = [_i, _i] + [_vp] * 4
         L.msod_closure_sensitivity.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d] + [_vp] * 5
         L.msod_closure_gradient.argtypes = [ctypes.c_int64] + [_vp] * 4
+        L.msod_closure_gradient_noalloc.argtypes = [ctypes.c_int64] + [_vp] * 4
         L.msod_closure_adam.argtypes = [_vp] * 4 + [_i, _d, _d, _d, _d]
         L.msod_closure_loss.argtypes = [_i] + [_vp] * 6
         L.msod_elem_jacobian.argtypes = [_i, _i, _i] + [_vp] * 7

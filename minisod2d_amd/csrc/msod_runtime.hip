@@ -26,6 +26,15 @@
 #include <atomic>
 #include <vector>
 
+// Partial sums of msod_closure_gradient_noalloc, which allocates nothing:
+// one row of kMlpTrain floats per workgroup of the gradient kernel, at most
+// kNoAllocBlocks workgroups (1024 x 1505 floats = 6.2 MB per GPU, held until
+// the process ends).  Host code never uses this name as an address: the
+// device address of each GPU's copy is taken once per device in msod_init
+// (hipGetSymbolAddress) and used from then on.
+constexpr int kNoAllocBlocks = 1024;
+static __device__ __align__(256) float scratch[kNoAllocBlocks * msod::kMlpTrain];
+
 // roctx range around each entry point: the counterpart of the reference's
 // nvtxStartRange/nvtxEndRange inside both kernels
 // (src/utils/sources/mod_nvtx.f90:54-80, elem_convec.f90:221,429,
@@ -312,7 +321,9 @@ struct State {
 
 State g;
 std::recursive_mutex g_mu;
-
+// device address of `scratch` on each device, set by msod_init
+constexpr int kMaxDevices = 64;
+float *g_noalloc_scratch[kMaxDevices] = {};
 thread_local std::string g_err;
 
 int fail(int status, const char *fmt, ...)
@@ -1073,6 +1084,11 @@ int msod_init(int device)
     HIPCHK(hipGetDeviceCount(&n), "hipGetDeviceCount");
     if (device < 0 || device >= n) return fail(MSOD_ERR_ARG, "device %d of %d", device, n);
     HIPCHK(hipSetDevice(device), "hipSetDevice");
+    if (device < kMaxDevices && !g_noalloc_scratch[device]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_noalloc_scratch[device] = (float *)ptr;
+    }
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking), "hipStreamCreate");
     g.device = device;
     g.init = true;
@@ -2066,6 +2082,49 @@ int msod_closure_gradient(int64_t nsamples, const float *mlp_params, const float
            "closure gradient launch");
     tk.done();
     return finish_call("closure gradient");
+}
+
+int msod_closure_gradient_noalloc(int64_t nsamples, const float *mlp_params, const float *features,
+                                  const float *gnu, float *grad)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_closure_gradient_noalloc");
+    NEED_INIT();
+    if (nsamples <= 0) return fail(MSOD_ERR_ARG, "nsamples=%lld", (long long)nsamples);
+    if (g.device >= kMaxDevices || !g_noalloc_scratch[g.device])
+        return fail(MSOD_ERR_STATE, "no partial-sum array on device %d", g.device);
+    const int64_t nblocks = std::min<int64_t>((nsamples + 63) / 64, kNoAllocBlocks);
+    if (nblocks > kNoAllocBlocks)  // the workgroups stride over the samples: never taken
+        return fail(MSOD_ERR_ARG, "closure gradient: %lld workgroups exceed the %d rows of the partial-sum array",
+                    (long long)nblocks, kNoAllocBlocks);
+    // the weights must already be on the device: a host block would need a
+    // library buffer
+    const void *dmlp, *df, *dg, *dgr;
+    uint64_t vx;
+    uintptr_t base;
+    if (!mlp_params || !(find_region(mlp_params, &base) || find_region_dev(mlp_params) ||
+                         is_device_pointer(mlp_params)))
+        return fail(MSOD_ERR_NOT_PRESENT, "mlp_params must be present or device memory");
+    int st;
+    if ((st = resolve(mlp_params, sizeof(float) * kMlpParams, "mlp_params", &dmlp, &vx))) return st;
+    float h[kMlpParams];  // usability check on the host, without a heap buffer
+    HIPCHK(hipMemcpyAsync(h, dmlp, sizeof h, hipMemcpyDeviceToHost, g.stream), "reading mlp_params");
+    HIPCHK(hipStreamSynchronize(g.stream), "reading mlp_params");
+    bool ok = h[kOScale] >= 0.f;
+    for (float v : h) ok = ok && std::isfinite(v);
+    if (!ok) return fail(MSOD_ERR_MODEL, "mlp_params not finite or with a negative scale");
+    if ((st = resolve(features, sizeof(float) * kMlpIn * nsamples, "features", &df, &vx))) return st;
+    if ((st = resolve(gnu, sizeof(float) * nsamples, "gnu", &dg, &vx))) return st;
+    if ((st = resolve(grad, sizeof(float) * kMlpTrain, "grad", &dgr, &vx))) return st;
+    // every workgroup overwrites its whole row before the reduction reads
+    // it, so the rows left by an earlier call need no clearing
+    HIPCHK(launch_mlp_grad((const float *)dmlp, (const float *)df, (const float *)dg, nsamples,
+                           g_noalloc_scratch[g.device], (int)nblocks, (float *)dgr, g.stream),
+           "closure gradient launch");
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hipfail(e, "closure gradient");
+    if (!g.async) HIPCHK(hipStreamSynchronize(g.stream), "closure gradient");
+    return MSOD_OK;
 }
 
 int msod_closure_adam(float *mlp_params, const float *grad, float *m, float *v, int step,
