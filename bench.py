@@ -153,7 +153,14 @@ def minimal_flops(E, q):
 TIMERS = {"fp64": (m.TIMER_CONVEC, m.TIMER_DIFFU),
           "mixed": (m.TIMER_CONVEC_MIXED, m.TIMER_DIFFU_MIXED),
           "geo32": (m.TIMER_CONVEC_GEO32, m.TIMER_DIFFU_GEO32),
-          "fp32": (m.TIMER_CONVEC_FP32, m.TIMER_DIFFU_FP32)}
+          "fp32": (m.TIMER_CONVEC_FP32, m.TIMER_DIFFU_FP32),
+          "f32split": (m.TIMER_CONVEC_F32SPLIT, m.TIMER_DIFFU_F32SPLIT)}
+
+
+def split_bytes(E, N):
+    """Canonical bytes of the FP32 split (FP32 He, gpvol, mu_e, mu_sgs, q,
+    rho, E, Tem, mu_fluid; FP64 u and pr; FP64 outputs)."""
+    return E * (4 + 10 * 4) + N * (3 * 8 + 8 + 3 * 4 + 4 + 4 + 40), E * (4 + 10 * 4 + 2 * 4) + N * (3 * 8 + 3 * 4 + 40)
 
 
 def run_pair(prob, R, D, variant, f32=None):
@@ -167,6 +174,14 @@ def run_pair(prob, R, D, variant, f32=None):
         diff(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"], t.dlxigp_ip,
              t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp, prob.Pr, f32["rho"], f32["u"],
              f32["Tem"], f32["mu_fluid"], f32["mu_e"], f32["mu_sgs"], *D)
+    elif variant == "f32split":
+        m.full_convec_ijk_f32split(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
+                                   t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.u, f32["q"],
+                                   f32["rho"], prob.pr, f32["E"], *R)
+        m.full_diffusion_ijk_f32split(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
+                                      t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp, prob.Pr,
+                                      f32["rho"], prob.u, f32["Tem"], f32["mu_fluid"], f32["mu_e"],
+                                      f32["mu_sgs"], *D)
     elif variant == "geo32":
         m.full_convec_ijk_geo32(prob.nelem, prob.npoin, prob.connec, f32["He"], f32["gpvol"],
                                 t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.u, prob.q,
@@ -400,6 +415,18 @@ def sampled_parity(prob, out, nsample=3000, seed=7):
     return errs
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(porder, n=32, nruns=3):
     """The reference's own Fortran kernels (oracle/_ref, compiled from the
     reference sources, rp=8, single thread as the reference is) on an
@@ -428,7 +455,7 @@ def cpu_baseline(porder, n=32, nruns=3):
         td.append(t2 - t1)
     tcm, tdm = float(np.median(tc)), float(np.median(td))
     return dict(value=prob.npoin / (tcm + tdm), unit="GLL-node updates/s", cores=1,
-                kind="reference",
+                kind="reference", cpu_model=cpu_model(), nproc=os.cpu_count(),
                 sample=f"{n}^3 elements porder {porder} ({prob.npoin} nodes), reference "
                        f"full_convec_ijk + full_diffusion_ijk (amdflang -O3, rp=8), median of "
                        f"{nruns}: convec {tcm*1e3:.1f} ms, diffu {tdm*1e3:.1f} ms")
@@ -517,7 +544,7 @@ def main():
     if not args.no_mixed:
         f32 = prob.fp32()
         m.copyin(*[f32[k] for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")])
-        for variant in ("mixed", "geo32", "fp32"):
+        for variant in ("f32split", "mixed", "geo32", "fp32"):
             log(f"timing {variant}")
             r = timed(prob, R, D, args.steps, args.warmup, variant, f32=f32)
             log(f"{variant} {r}")
@@ -562,7 +589,9 @@ def main():
                                  "ms_per_step": other["ms_per_step"],
                                  "kernels_ms": {"convec": other["convec_ms"], "diffu": other["diffu_ms"],
                                                 "zero_fill": other["zero_ms"]}}
-    notes = {"mixed": "every input stored in FP32, FP64 arithmetic and assembly; error = FP32 "
+    notes = {"f32split": "FP32 He, gpvol, mu_e, mu_sgs, q, rho, E, Tem, mu_fluid; FP64 u and pr "
+                         "(chosen field by field for <= 1e-6); FP64 arithmetic and atomic assembly",
+             "mixed": "every input stored in FP32, FP64 arithmetic and assembly; error = FP32 "
                       "rounding of the inputs, amplified by the derivative sweeps",
              "geo32": "He, gpvol, mu_e, mu_sgs stored in FP32, nodal fields FP64, FP64 "
                       "arithmetic and assembly",
@@ -570,13 +599,14 @@ def main():
                      "FP64 node sums, atomic assembly and pressure recovery"}
     for variant, r in split.items():
         wb = (4, 8, 4) if variant == "geo32" else (4, 4, 4)
-        bcm, bdm = canonical_bytes(E, N, *wb)
+        bcm, bdm = split_bytes(E, N) if variant == "f32split" else canonical_bytes(E, N, *wb)
         rec[variant] = {"value": N / (r["ms_per_step"] * 1e-3), "ms_per_step": r["ms_per_step"],
                         "speedup_vs_fp64": r64["ms_per_step"] / r["ms_per_step"],
                         "kernels_ms": {"convec": r["convec_ms"], "diffu": r["diffu_ms"]},
                         "hbm_frac_convec": bcm / (r["convec_ms"] * 1e-3) / HBM_PEAK,
                         "hbm_frac_diffu": bdm / (r["diffu_ms"] * 1e-3) / HBM_PEAK,
                         "rel_l2_vs_fp64_oracle": r.get("sampled_rel_l2"),
+                        "max_rel_l2": (max(r["sampled_rel_l2"].values()) if r.get("sampled_rel_l2") else None),
                         "storage": notes[variant]}
     m.finalize()
     del prob, R, D, ins

@@ -109,8 +109,9 @@ int msod_timing_enable(int enable);
  *         generator (msod_box_mesh, msod_flow_fields,
  *         msod_viscosity_fields), 11 TET convec (msod_fem_convec),
  *         12 TET diffu (msod_fem_diffu), 13 TET geometry
- *         (msod_tet_jacobian), 14 convec fp32, 15 diffu fp32.  Returns
- *         accumulated ms and launch count. */
+ *         (msod_tet_jacobian), 14 convec fp32, 15 diffu fp32, 16 convec
+ *         rp=4 drop-in, 17 diffu rp=4 drop-in, 18 convec f32split, 19 diffu
+ *         f32split.  Returns accumulated ms and launch count. */
 int msod_timing_get(int kernel, double *total_ms, int64_t *launches);
 int msod_timing_reset(void);
 
@@ -243,6 +244,48 @@ int msod_full_diffusion_ijk_fp32(int porder, int nelem, int npoin, const int32_t
                                  double Pr, const float *rho, const float *u, const float *Tem,
                                  const float *mu_fluid, const float *mu_e, const float *mu_sgs,
                                  double *Rmass, double *Rmom, double *Rener);
+
+/*   _f32split: the FP32 split at the accuracy of the FP64 form's inputs:
+ *     every input in FP32 except the velocity u and the pressure pr, which
+ *     stay FP64; FP64 arithmetic, node sums and outputs.  The choice is
+ *     measured field by field: rounding u to FP32 alone puts the diffusive
+ *     momentum residual at 2e-5 (u is differentiated twice, the error grows
+ *     like 1/h), rounding pr alone the convective momentum residual at 1e-6;
+ *     each other field costs at most 6e-7 of relative L2 on any residual.
+ *     Canonical bytes per pair: 1.57x fewer than FP64. */
+int msod_full_convec_ijk_f32split(int porder, int nelem, int npoin, const int32_t *connec, const float *He,
+                                  const float *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                                  const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                                  const double *u, const float *q, const float *rho, const double *pr,
+                                  const float *E, double *Rmass, double *Rmom, double *Rener);
+int msod_full_diffusion_ijk_f32split(int porder, int nelem, int npoin, const int32_t *connec, const float *He,
+                                     const float *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                                     const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                                     double Cp, double Pr, const float *rho, const double *u, const float *Tem,
+                                     const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                     double *Rmass, double *Rmom, double *Rener);
+
+/* The drop-in for a caller built at the reference's default real kind
+ * (rp = 4, src/utils/sources/mod_constants.f90:5): full_convec_ijk and
+ * full_diffusion_ijk with the reference's full argument lists
+ * (elem_convec.f90:198-211, elem_diffu.f90:10-23) in float, Cp and Pr
+ * included, and float outputs.  The kernels compute in FP64 from the float
+ * inputs and round each node's sum once to float, so the only errors are the
+ * float storage of inputs and outputs (the reference at rp = 4 also rounds
+ * inside every product and sum). */
+int msod_full_convec_ijk_r4(int porder, int nelem, int npoin, const int32_t *connec, const float *Ngp,
+                            const float *dNgp, const float *He, const float *gpvol,
+                            const float *dlxigp_ip, const float *xgp, const int32_t *invAtoIJK,
+                            const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                            const float *u, const float *q, const float *rho, const float *pr,
+                            const float *E, float *Rmass, float *Rmom, float *Rener);
+int msod_full_diffusion_ijk_r4(int porder, int nelem, int npoin, const int32_t *connec, const float *Ngp,
+                               const float *dNgp, const float *He, const float *gpvol,
+                               const float *dlxigp_ip, const float *xgp, const int32_t *invAtoIJK,
+                               const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                               float Cp, float Pr, const float *rho, const float *u, const float *Tem,
+                               const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                               float *Rmass, float *Rmom, float *Rener);
 
 /* ---- learned subgrid-viscosity closure ----------------------------------
  * No counterpart in the reference: its driver sets mu_sgs = 1

@@ -42,7 +42,8 @@ MSOD_ERR_MODEL = 8
 (TIMER_ZERO, TIMER_CONVEC, TIMER_DIFFU, TIMER_CONVEC_MIXED, TIMER_DIFFU_MIXED, TIMER_DIFFU_MLP,
  TIMER_CLOSURE_TRAIN, TIMER_CONVEC_GEO32, TIMER_DIFFU_GEO32, TIMER_GEOMETRY,
  TIMER_GENERATOR, TIMER_FEM_CONVEC, TIMER_FEM_DIFFU, TIMER_TET_GEOMETRY, TIMER_CONVEC_FP32,
- TIMER_DIFFU_FP32) = range(16)
+ TIMER_DIFFU_FP32, TIMER_CONVEC_R4, TIMER_DIFFU_R4, TIMER_CONVEC_F32SPLIT,
+ TIMER_DIFFU_F32SPLIT) = range(20)
 
 # the C-ABI entry points declared in include/msod2d.h (checked by the tests)
 ABI_SYMBOLS = (
@@ -59,7 +60,8 @@ ABI_SYMBOLS = (
     "msod_closure_loss", "msod_elem_jacobian", "msod_box_mesh", "msod_flow_fields",
     "msod_viscosity_fields", "msod_tet_jacobian", "msod_fem_convec", "msod_fem_diffu",
     "msod_full_convec_ijk_fp32", "msod_full_diffusion_ijk_fp32", "msod_data_register",
-    "msod_closure_gradient_noalloc",
+    "msod_closure_gradient_noalloc", "msod_full_convec_ijk_r4", "msod_full_diffusion_ijk_r4",
+    "msod_full_convec_ijk_f32split", "msod_full_diffusion_ijk_f32split",
 )
 
 
@@ -110,6 +112,11 @@ def hip_lib() -> ctypes.CDLL:
         L.msod_full_convec_ijk_fp32.argtypes = [_i, _i, _i] + [_vp] * 12 + [_d] + [_vp] * 3
         L.msod_full_diffusion_ijk_fp32.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
         L.msod_full_diffusion_ijk_geo32.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
+        L.msod_full_convec_ijk_f32split.argtypes = [_i, _i, _i] + [_vp] * 16
+        L.msod_full_diffusion_ijk_f32split.argtypes = [_i, _i, _i] + [_vp] * 8 + [_d, _d] + [_vp] * 9
+        L.msod_full_convec_ijk_r4.argtypes = [_i, _i, _i] + [_vp] * 19
+        L.msod_full_diffusion_ijk_r4.argtypes = ([_i, _i, _i] + [_vp] * 11 + [ctypes.c_float] * 2 +
+                                                 [_vp] * 9)
         L.msod_full_diffusion_ijk_closure.argtypes = ([_i, _i, _i] + [_vp] * 11 + [_d, _d] +
                                                       [_vp] * 10)
         L.msod_closure_features.argtypes = [_i, _i, _i] + [_vp] * 11
@@ -319,6 +326,45 @@ def full_convec_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAto
         _porder(invAtoIJK), nelem, npoin,
         *map(_ptr, (connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK,
                     u32, q32, rho32, E32)), float(gamma_gas), *map(_ptr, (Rmass, Rmom, Rener))))
+
+
+def full_convec_ijk_f32split(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                             gmshAtoJ, gmshAtoK, u, q32, rho32, pr, E32, Rmass, Rmom, Rener):
+    """FP32 split: FP32 He, gpvol, q, rho, E; FP64 u and pr; FP64 outputs."""
+    _check(hip_lib().msod_full_convec_ijk_f32split(
+        _porder(invAtoIJK), nelem, npoin, *map(_ptr, (
+            connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, q32, rho32,
+            pr, E32, Rmass, Rmom, Rener))))
+
+
+def full_diffusion_ijk_f32split(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,
+                                gmshAtoJ, gmshAtoK, Cp, Pr, rho32, u, Tem32, mu_fluid32, mu_e32,
+                                mu_sgs32, Rmass, Rmom, Rener):
+    """FP32 split: FP32 He, gpvol, rho, Tem, mu_fluid, mu_e, mu_sgs; FP64 u; FP64 outputs."""
+    _check(hip_lib().msod_full_diffusion_ijk_f32split(
+        _porder(invAtoIJK), nelem, npoin, *map(_ptr, (
+            connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
+        float(Cp), float(Pr), *map(_ptr, (rho32, u, Tem32, mu_fluid32, mu_e32, mu_sgs32, Rmass, Rmom,
+                                          Rener))))
+
+
+def full_convec_ijk_r4(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK,
+                       gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener):
+    """full_convec_ijk for a caller built at rp=4: float32 arrays and outputs."""
+    _check(hip_lib().msod_full_convec_ijk_r4(
+        _porder(invAtoIJK), nelem, npoin, *map(_ptr, (
+            connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ,
+            gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener))))
+
+
+def full_diffusion_ijk_r4(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK,
+                          gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs,
+                          Rmass, Rmom, Rener):
+    """full_diffusion_ijk for a caller built at rp=4: float32 arrays and outputs."""
+    _check(hip_lib().msod_full_diffusion_ijk_r4(
+        _porder(invAtoIJK), nelem, npoin, *map(_ptr, (
+            connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK)),
+        float(Cp), float(Pr), *map(_ptr, (rho, u, Tem, mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener))))
 
 
 def full_diffusion_ijk_mixed(nelem, npoin, connec, He32, gpvol32, dlxigp_ip, invAtoIJK, gmshAtoI,

@@ -35,15 +35,16 @@ namespace msod {
 
 // Inputs of one element node, loaded one pass ahead of their use so that
 // the gathers of the next pass are in flight while this pass computes.
-template <typename TF>
+template <class A>
 struct ConvecNode {
-    TF r, E, p, u[3], q[3];
+    typename A::F r, E, q[3];
+    typename A::U u[3];
+    typename A::Pt p;
     uint32_t li;
 };
 
-template <int NN, typename TG, typename TF>
-__device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_t e, int l, uint32_t gnode,
-                                             ConvecNode<TF> &x)
+template <int NN, class A>
+__device__ __forceinline__ void convec_fetch(const A &a, int64_t e, int l, uint32_t gnode, ConvecNode<A> &x)
 {
     const int64_t g = gnode;
     const int64_t np = a.npoin;
@@ -54,7 +55,7 @@ __device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_
         x.u[d] = a.u[g + d * np];
         x.q[d] = a.q[g + d * np];
     }
-    x.p = a.pr ? a.pr[g] : (TF)0;
+    x.p = a.pr ? a.pr[g] : (typename A::Pt)0;
     x.li = __builtin_nontemporal_load(&a.plan.lidx[e * NN + l]);
 }
 
@@ -62,9 +63,10 @@ __device__ __forceinline__ void convec_fetch(const ConvecArgs<TG, TF> &a, int64_
 // flight; 4 would force spills.  TA: arithmetic type of the gathers, fluxes
 // and sweeps (FP64, or FP32 for the FP32-arithmetic form); the node sums and
 // the assembly stay FP64.
-template <int P, typename TG, typename TF, typename TA = double>
-__global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a)
+template <int P, class A, typename TA = double>
+__global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(A a)
 {
+    using TG = typename A::G;
     using S = Shape<P>;
     using T = TA;  // arithmetic
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX, NPASS = S::NPASS;
@@ -107,7 +109,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
     for (int i = t; i < Q * Q; i += blockDim.x) sD[i] = (T)a.tab.D[i];
 
     // prologue: nodal values of pass 0, connectivity of pass 1
-    ConvecNode<TF> nd[2];
+    ConvecNode<A> nd[2];
     uint32_t gnext;  // global node of the next pass's element node
     {
         const int64_t b0 = pl.batches[first];
@@ -133,8 +135,8 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
             for (int c = 0; c < 5; ++c) sR[c][i] = 0.0;
 #pragma unroll
         for (int p = 0; p < NPASS; ++p) {
-            ConvecNode<TF> &cur = nd[p & 1];
-            ConvecNode<TF> &nxt = nd[(p + 1) & 1];
+            ConvecNode<A> &cur = nd[p & 1];
+            ConvecNode<A> &nxt = nd[(p + 1) & 1];
             const int g = kb * NPASS + p;
             const int64_t e = b * S::BATCH + p * EPB + le;
             const bool active = (le < EPB) && (e < a.nelem);
@@ -252,26 +254,26 @@ __global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(ConvecArgs<TG, TF> a
     }
 }
 
-template <int P, typename TG, typename TF, typename TA = double>
-static hipError_t launch_p(const ConvecArgs<TG, TF> &a, hipStream_t s)
+template <int P, class A, typename TA = double>
+static hipError_t launch_p(const A &a, hipStream_t s)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_convec<P, TG, TF, TA>), dim3(plan_blocks(a.plan)), dim3(Shape<P>::NT), 0, s, a);
+    hipLaunchKernelGGL((k_convec<P, A, TA>), dim3(plan_blocks(a.plan)), dim3(Shape<P>::NT), 0, s, a);
     return hipGetLastError();
 }
 
-template <typename TG, typename TF>
-hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s)
+template <class A>
+hipError_t launch_convec(int porder, const A &a, hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, TG, TF>(a, s);
-    case 2: return launch_p<2, TG, TF>(a, s);
-    case 3: return launch_p<3, TG, TF>(a, s);
-    case 4: return launch_p<4, TG, TF>(a, s);
-    case 5: return launch_p<5, TG, TF>(a, s);
-    case 6: return launch_p<6, TG, TF>(a, s);
-    case 7: return launch_p<7, TG, TF>(a, s);
-    case 8: return launch_p<8, TG, TF>(a, s);
+    case 1: return launch_p<1, A>(a, s);
+    case 2: return launch_p<2, A>(a, s);
+    case 3: return launch_p<3, A>(a, s);
+    case 4: return launch_p<4, A>(a, s);
+    case 5: return launch_p<5, A>(a, s);
+    case 6: return launch_p<6, A>(a, s);
+    case 7: return launch_p<7, A>(a, s);
+    case 8: return launch_p<8, A>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -279,20 +281,22 @@ hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s)
 hipError_t launch_convec_fp32(int porder, const ConvecArgs<float, float> &a, hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, float, float, float>(a, s);
-    case 2: return launch_p<2, float, float, float>(a, s);
-    case 3: return launch_p<3, float, float, float>(a, s);
-    case 4: return launch_p<4, float, float, float>(a, s);
-    case 5: return launch_p<5, float, float, float>(a, s);
-    case 6: return launch_p<6, float, float, float>(a, s);
-    case 7: return launch_p<7, float, float, float>(a, s);
-    case 8: return launch_p<8, float, float, float>(a, s);
+    case 1: return launch_p<1, ConvecArgs<float, float>, float>(a, s);
+    case 2: return launch_p<2, ConvecArgs<float, float>, float>(a, s);
+    case 3: return launch_p<3, ConvecArgs<float, float>, float>(a, s);
+    case 4: return launch_p<4, ConvecArgs<float, float>, float>(a, s);
+    case 5: return launch_p<5, ConvecArgs<float, float>, float>(a, s);
+    case 6: return launch_p<6, ConvecArgs<float, float>, float>(a, s);
+    case 7: return launch_p<7, ConvecArgs<float, float>, float>(a, s);
+    case 8: return launch_p<8, ConvecArgs<float, float>, float>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_convec<double, double>(int, const ConvecArgs<double, double> &, hipStream_t);
-template hipError_t launch_convec<float, float>(int, const ConvecArgs<float, float> &, hipStream_t);
-template hipError_t launch_convec<float, double>(int, const ConvecArgs<float, double> &, hipStream_t);
+template hipError_t launch_convec(int, const ConvecArgs<double, double> &, hipStream_t);
+template hipError_t launch_convec(int, const ConvecArgs<float, float> &, hipStream_t);
+template hipError_t launch_convec(int, const ConvecArgs<float, double> &, hipStream_t);
+template hipError_t launch_convec(int, const ConvecArgs<float, float, float> &, hipStream_t);
+template hipError_t launch_convec(int, const ConvecSplitArgs &, hipStream_t);
 
 }  // namespace msod

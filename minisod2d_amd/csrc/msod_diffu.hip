@@ -38,14 +38,14 @@ namespace msod {
 
 // Inputs of one element node, loaded one pass ahead of their use so that
 // the gathers of the next pass are in flight while this pass computes.
-template <typename TG, typename TF>
+template <class A>
 struct DiffuNode {
-    TF r, T_, muf, u[3];
+    typename A::F r, T_, muf;
+    typename A::U u[3];
 };
 
-template <int NN, bool CLOSURE, typename TG, typename TF>
-__device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t e, int l,
-                                            DiffuNode<TG, TF> &x)
+template <int NN, bool CLOSURE, class A>
+__device__ __forceinline__ void diffu_fetch(const A &a, int64_t e, int l, DiffuNode<A> &x)
 {
     const int64_t g = __builtin_nontemporal_load(&a.plan.conn[e * NN + l]);
     const int64_t np = a.npoin;
@@ -62,8 +62,8 @@ __device__ __forceinline__ void diffu_fetch(const DiffuArgs<TG, TF> &a, int64_t 
 // lines (and pages) per load; here ME neighbouring lanes read ME
 // consecutive elements of one node, and the batches that follow in mesh
 // order reuse the lines.
-template <int NN, int ME, int NT, bool CLOSURE, typename TG, typename TF>
-__device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64_t e0, TG (*sMu)[ME][NN])
+template <int NN, int ME, int NT, bool CLOSURE, class A, typename TG = typename A::G>
+__device__ __forceinline__ void diffu_stage_mu(const A &a, int64_t e0, TG (*sMu)[ME][NN])
 {
     // every load unconditional and issued before any LDS write (entries past
     // the table or the mesh re-read a valid entry and are not written or not
@@ -97,9 +97,10 @@ __device__ __forceinline__ void diffu_stage_mu(const DiffuArgs<TG, TF> &a, int64
 // The closure form is bounded to 3 workgroups per CU: the compiler then keeps
 // the MFMA accumulators in VGPRs (about 148 in all) instead of 188 VGPRs and
 // 32 AGPRs, 3 waves per SIMD instead of 2 (4.7 against 6.2 ms at 96^3 p=3).
-template <int P, typename TG, typename TF, bool CLOSURE, typename TA = double>
-__global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ? 3 : 1) k_diffu(DiffuArgs<TG, TF> a, const float *mlp)
+template <int P, class A, bool CLOSURE, typename TA = double>
+__global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ? 3 : 1) k_diffu(A a, const float *mlp)
 {
+    using TG = typename A::G;
     using S = Shape<P>;
     using T = TA;  // arithmetic: FP64, or FP32 for the FP32-arithmetic form (node sums stay FP64)
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX;
@@ -145,7 +146,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
     const int I = l % Q, J = (l / Q) % Q, K = l / (Q * Q);
     const T Cp = (T)a.Cp, Pr = (T)a.Pr;
 
-    DiffuNode<TG, TF> cur, nxt;
+    DiffuNode<A> cur, nxt;
     static_assert(Q * Q <= S::NT, "one derivative-table entry per thread");
     {
         // prologue: the tables and the connectivity first, then the values
@@ -332,27 +333,27 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
     write_batch_nodes<UMAX, NT>(pl.mode, wn, U, sR, a.Rmass, a.Rmom, a.Rener, a.npoin);
 }
 
-template <int P, typename TG, typename TF, bool CLOSURE = false, typename TA = double>
-static hipError_t launch_p(const DiffuArgs<TG, TF> &a, hipStream_t s, const float *mlp = nullptr)
+template <int P, class A, bool CLOSURE = false, typename TA = double>
+static hipError_t launch_p(const A &a, hipStream_t s, const float *mlp = nullptr)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_diffu<P, TG, TF, CLOSURE, TA>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s,
+    hipLaunchKernelGGL((k_diffu<P, A, CLOSURE, TA>), dim3((unsigned)a.plan.nb), dim3(Shape<P>::NT), 0, s,
                        a, mlp);
     return hipGetLastError();
 }
 
-template <typename TG, typename TF>
-hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s)
+template <class A>
+hipError_t launch_diffu(int porder, const A &a, hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, TG, TF>(a, s);
-    case 2: return launch_p<2, TG, TF>(a, s);
-    case 3: return launch_p<3, TG, TF>(a, s);
-    case 4: return launch_p<4, TG, TF>(a, s);
-    case 5: return launch_p<5, TG, TF>(a, s);
-    case 6: return launch_p<6, TG, TF>(a, s);
-    case 7: return launch_p<7, TG, TF>(a, s);
-    case 8: return launch_p<8, TG, TF>(a, s);
+    case 1: return launch_p<1, A>(a, s);
+    case 2: return launch_p<2, A>(a, s);
+    case 3: return launch_p<3, A>(a, s);
+    case 4: return launch_p<4, A>(a, s);
+    case 5: return launch_p<5, A>(a, s);
+    case 6: return launch_p<6, A>(a, s);
+    case 7: return launch_p<7, A>(a, s);
+    case 8: return launch_p<8, A>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -360,34 +361,36 @@ hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s)
 hipError_t launch_diffu_fp32(int porder, const DiffuArgs<float, float> &a, hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, float, float, false, float>(a, s);
-    case 2: return launch_p<2, float, float, false, float>(a, s);
-    case 3: return launch_p<3, float, float, false, float>(a, s);
-    case 4: return launch_p<4, float, float, false, float>(a, s);
-    case 5: return launch_p<5, float, float, false, float>(a, s);
-    case 6: return launch_p<6, float, float, false, float>(a, s);
-    case 7: return launch_p<7, float, float, false, float>(a, s);
-    case 8: return launch_p<8, float, float, false, float>(a, s);
+    case 1: return launch_p<1, DiffuArgs<float, float>, false, float>(a, s);
+    case 2: return launch_p<2, DiffuArgs<float, float>, false, float>(a, s);
+    case 3: return launch_p<3, DiffuArgs<float, float>, false, float>(a, s);
+    case 4: return launch_p<4, DiffuArgs<float, float>, false, float>(a, s);
+    case 5: return launch_p<5, DiffuArgs<float, float>, false, float>(a, s);
+    case 6: return launch_p<6, DiffuArgs<float, float>, false, float>(a, s);
+    case 7: return launch_p<7, DiffuArgs<float, float>, false, float>(a, s);
+    case 8: return launch_p<8, DiffuArgs<float, float>, false, float>(a, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_diffu<double, double>(int, const DiffuArgs<double, double> &, hipStream_t);
-template hipError_t launch_diffu<float, float>(int, const DiffuArgs<float, float> &, hipStream_t);
-template hipError_t launch_diffu<float, double>(int, const DiffuArgs<float, double> &, hipStream_t);
+template hipError_t launch_diffu(int, const DiffuArgs<double, double> &, hipStream_t);
+template hipError_t launch_diffu(int, const DiffuArgs<float, float> &, hipStream_t);
+template hipError_t launch_diffu(int, const DiffuArgs<float, double> &, hipStream_t);
+template hipError_t launch_diffu(int, const DiffuArgs<float, float, float> &, hipStream_t);
+template hipError_t launch_diffu(int, const DiffuSplitArgs &, hipStream_t);
 
 hipError_t launch_diffu_closure(int porder, const DiffuArgs<double, double> &a, const float *mlp,
-                                hipStream_t s)
+                                        hipStream_t s)
 {
     switch (porder) {
-    case 1: return launch_p<1, double, double, true>(a, s, mlp);
-    case 2: return launch_p<2, double, double, true>(a, s, mlp);
-    case 3: return launch_p<3, double, double, true>(a, s, mlp);
-    case 4: return launch_p<4, double, double, true>(a, s, mlp);
-    case 5: return launch_p<5, double, double, true>(a, s, mlp);
-    case 6: return launch_p<6, double, double, true>(a, s, mlp);
-    case 7: return launch_p<7, double, double, true>(a, s, mlp);
-    case 8: return launch_p<8, double, double, true>(a, s, mlp);
+    case 1: return launch_p<1, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 2: return launch_p<2, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 3: return launch_p<3, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 4: return launch_p<4, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 5: return launch_p<5, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 6: return launch_p<6, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 7: return launch_p<7, DiffuArgs<double, double>, true>(a, s, mlp);
+    case 8: return launch_p<8, DiffuArgs<double, double>, true>(a, s, mlp);
     default: return hipErrorInvalidValue;
     }
 }

@@ -87,34 +87,59 @@ inline unsigned plan_blocks(const Plan &p) { return (unsigned)((p.nb + p.kpb - 1
 //   <double, double>  FP64 form (the reference's operands)
 //   <float,  float>   FP32 storage of every input
 //   <float,  double>  FP32 element arrays, FP64 nodal fields
-template <typename TG, typename TF>
+// TO: type of the outputs, FP64, or FP32 for callers built at the
+// reference's default real kind (rp = 4); the sums before the output stay
+// FP64.  TU, TP: storage of the velocity and of the pressure where it differs
+// from the other nodal fields (the FP32 split keeps both in FP64: the
+// velocity is differentiated twice in the diffusive momentum residual, the
+// pressure gradient dominates the convective one).  The kernels take the
+// argument type as a template parameter and read the storage types from it.
+template <typename TG, typename TF, typename TO = double, typename TU = TF, typename TP = TF>
 struct ConvecArgs {
+    using G = TG;
+    using F = TF;
+    using O = TO;
+    using U = TU;
+    using Pt = TP;
     int64_t nelem, npoin;
     Plan plan;
     const TG *geo, *gvol;
-    const TF *u, *q, *rho, *pr, *E;  // pr == nullptr: recover pressure from E
+    const TU *u;
+    const TF *q, *rho, *E;
+    const TP *pr;                    // nullptr: recover pressure from E
     double gamma_m1;                 // gamma-1, used when pr == nullptr
-    double *Rmass, *Rmom, *Rener;
+    TO *Rmass, *Rmom, *Rener;
     Tables tab;
 };
 
-template <typename TG, typename TF>
+template <typename TG, typename TF, typename TO = double, typename TU = TF>
 struct DiffuArgs {
+    using G = TG;
+    using F = TF;
+    using O = TO;
+    using U = TU;
     int64_t nelem, npoin;
     Plan plan;
     const TG *geo, *gvol;
-    const TF *rho, *u, *Tem, *mu_fluid;
+    const TF *rho, *Tem, *mu_fluid;
+    const TU *u;
     const TG *mu_e, *mu_sgs;
     double Cp, Pr, c_rho, c_ener;
-    double *Rmass, *Rmom, *Rener;
+    TO *Rmass, *Rmom, *Rener;
     Tables tab;
 };
 
-// Launchers (return hipError_t of the launch).
-template <typename TG, typename TF>
-hipError_t launch_convec(int porder, const ConvecArgs<TG, TF> &a, hipStream_t s);
-template <typename TG, typename TF>
-hipError_t launch_diffu(int porder, const DiffuArgs<TG, TF> &a, hipStream_t s);
+// The FP32 split: FP32 geometry and nodal fields, FP64 velocity (and
+// pressure), FP64 arithmetic and outputs.
+using ConvecSplitArgs = ConvecArgs<float, float, double, double, double>;
+using DiffuSplitArgs = DiffuArgs<float, float, double, double>;
+
+// Launchers (return hipError_t of the launch); A is a ConvecArgs or
+// DiffuArgs instantiation.
+template <class A>
+hipError_t launch_convec(int porder, const A &a, hipStream_t s);
+template <class A>
+hipError_t launch_diffu(int porder, const A &a, hipStream_t s);
 // FP32 storage and FP32 gathers, fluxes and derivative sweeps; FP64 node sums,
 // assembly and (convection) pressure recovery.
 hipError_t launch_convec_fp32(int porder, const ConvecArgs<float, float> &a, hipStream_t s);
@@ -123,9 +148,11 @@ hipError_t launch_diffu_fp32(int porder, const DiffuArgs<float, float> &a, hipSt
 // Zero Rmass (n), Rmom (3n), Rener (n).
 hipError_t launch_zero_outputs(double *Rmass, double *Rmom, double *Rener, int64_t n,
                                hipStream_t s);
+hipError_t launch_zero_outputs(float *Rmass, float *Rmom, float *Rener, int64_t n, hipStream_t s);
 // Zero the listed nodes of the three outputs.
-hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
-                             const uint32_t *nodes, int64_t count, hipStream_t s);
+template <typename TO>
+hipError_t launch_zero_nodes(TO *Rmass, TO *Rmom, TO *Rener, int64_t npoin, const uint32_t *nodes,
+                             int64_t count, hipStream_t s);
 
 // Element-major geometry from the reference layouts (see top of file).
 template <typename T>
@@ -152,10 +179,10 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t i, uint32_t n)
 // plain store of the nodes no other batch holds was tried and is much
 // slower, since those lines also take atomics).  Coloured mode: the first
 // writer stores, the others add.
-template <int UMAX, int NT>
+template <int UMAX, int NT, typename TO>
 __device__ __forceinline__ void write_batch_nodes(int mode, const uint32_t (&wn)[(UMAX + NT - 1) / NT],
-                                                  uint32_t U, const double (*sR)[UMAX], double *Rmass,
-                                                  double *Rmom, double *Rener, int64_t npoin)
+                                                  uint32_t U, const double (*sR)[UMAX], TO *Rmass,
+                                                  TO *Rmom, TO *Rener, int64_t npoin)
 {
 #pragma unroll
     for (int j = 0; j < (UMAX + NT - 1) / NT; ++j) {
@@ -163,16 +190,16 @@ __device__ __forceinline__ void write_batch_nodes(int mode, const uint32_t (&wn)
         if (i >= U) break;
         const uint32_t w = wn[j];
         const int64_t g = w & 0x7fffffffu;
-        double *dst[5] = {Rmass + g, Rmom + g, Rmom + npoin + g, Rmom + 2 * npoin + g, Rener + g};
+        TO *dst[5] = {Rmass + g, Rmom + g, Rmom + npoin + g, Rmom + 2 * npoin + g, Rener + g};
         if (mode == kAtomic) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) unsafeAtomicAdd(dst[c], sR[c][i]);
+            for (int c = 0; c < 5; ++c) unsafeAtomicAdd(dst[c], (TO)sR[c][i]);
         } else if (w >> 31) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) *dst[c] = sR[c][i];
+            for (int c = 0; c < 5; ++c) *dst[c] = (TO)sR[c][i];
         } else {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) *dst[c] += sR[c][i];
+            for (int c = 0; c < 5; ++c) *dst[c] += (TO)sR[c][i];
         }
     }
 }

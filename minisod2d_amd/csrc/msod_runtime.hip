@@ -99,7 +99,25 @@ __global__ void k_zero_unaligned(double *p, int64_t n)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.0;
 }
 
-__global__ void k_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
+// Zero n floats: 16-byte stores over the 16-byte aligned body, single
+// stores for the head and tail.
+__global__ void k_zero_f32(float *p, int64_t n)
+{
+    const int64_t head = std::min<int64_t>(n, (int64_t)((16u - ((uintptr_t)p & 15u)) & 15u) / 4);
+    const int64_t body4 = (n - head) / 4;
+    const int64_t tail0 = head + body4 * 4;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    float4 *q = reinterpret_cast<float4 *>(p + head);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < body4; i += stride)
+        q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (blockIdx.x == 0 && threadIdx.x < 8) {
+        if (threadIdx.x < head) p[threadIdx.x] = 0.f;
+        if (tail0 + threadIdx.x < n) p[tail0 + threadIdx.x] = 0.f;
+    }
+}
+
+template <typename TO>
+__global__ void k_zero_nodes(TO *Rmass, TO *Rmom, TO *Rener, int64_t npoin,
                              const uint32_t *nodes, int64_t count)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -171,13 +189,31 @@ hipError_t launch_zero_doubles(double *p, int64_t n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_zero_nodes(double *Rmass, double *Rmom, double *Rener, int64_t npoin,
-                             const uint32_t *nodes, int64_t count, hipStream_t s)
+template <typename TO>
+hipError_t launch_zero_nodes(TO *Rmass, TO *Rmom, TO *Rener, int64_t npoin, const uint32_t *nodes,
+                             int64_t count, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zero_nodes, dim3(grid_for(count, 256)), dim3(256), 0, s, Rmass, Rmom, Rener,
+    hipLaunchKernelGGL(k_zero_nodes<TO>, dim3(grid_for(count, 256)), dim3(256), 0, s, Rmass, Rmom, Rener,
                        npoin, nodes, count);
     return hipGetLastError();
+}
+template hipError_t launch_zero_nodes<double>(double *, double *, double *, int64_t, const uint32_t *,
+                                              int64_t, hipStream_t);
+template hipError_t launch_zero_nodes<float>(float *, float *, float *, int64_t, const uint32_t *, int64_t,
+                                             hipStream_t);
+
+hipError_t launch_zero_outputs(float *Rmass, float *Rmom, float *Rener, int64_t n, hipStream_t s)
+{
+    const int64_t len[3] = {n, 3 * n, n};
+    float *ptr[3] = {Rmass, Rmom, Rener};
+    for (int k = 0; k < 3; ++k) {
+        const int64_t blocks = std::min<int64_t>(len[k] / 4 / 256 + 1, 0x7fffffff);
+        hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)blocks), dim3(256), 0, s, ptr[k], len[k]);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 int batch_size(int porder)
@@ -290,7 +326,7 @@ struct CheckedMesh {
     }
 };
 
-constexpr int kNumTimers = 16;
+constexpr int kNumTimers = 20;
 struct Pending {
     int kernel;
     hipEvent_t a, b;
@@ -439,8 +475,10 @@ int host_view(const void *p, size_t bytes, std::vector<char> &tmp, const char **
 }
 
 // Checks invAtoIJK/AtoI/J/K/dlxigp_ip and returns the cached device table set.
+// dl32: dlxigp_ip holds float (a caller built at rp = 4); its values are
+// widened to double exactly.
 int get_tables(int porder, const int32_t *invAtoIJK, const int32_t *AtoI, const int32_t *AtoJ,
-               const int32_t *AtoK, const double *dlxigp_ip, TableSet **out)
+               const int32_t *AtoK, const void *dlxigp_ip, TableSet **out, bool dl32 = false)
 {
     const int q = porder + 1, nn = q * q * q;
     std::vector<char> t0, t1, t2, t3, t4;
@@ -450,10 +488,17 @@ int get_tables(int porder, const int32_t *invAtoIJK, const int32_t *AtoI, const 
     if ((st = host_view(AtoI, sizeof(int32_t) * nn, t1, &pi, "gmshAtoI"))) return st;
     if ((st = host_view(AtoJ, sizeof(int32_t) * nn, t2, &pj, "gmshAtoJ"))) return st;
     if ((st = host_view(AtoK, sizeof(int32_t) * nn, t3, &pk, "gmshAtoK"))) return st;
-    if ((st = host_view(dlxigp_ip, sizeof(double) * nn * 3 * q, t4, &pdl, "dlxigp_ip"))) return st;
+    if ((st = host_view(dlxigp_ip, (dl32 ? sizeof(float) : sizeof(double)) * nn * 3 * q, t4, &pdl,
+                        "dlxigp_ip")))
+        return st;
     const int32_t *inv = (const int32_t *)pinv, *ai = (const int32_t *)pi,
                   *aj = (const int32_t *)pj, *ak = (const int32_t *)pk;
-    const double *dl = (const double *)pdl;
+    std::vector<double> dlw;
+    if (dl32) {
+        dlw.resize((size_t)nn * 3 * q);
+        for (size_t i = 0; i < dlw.size(); ++i) dlw[i] = (double)((const float *)pdl)[i];
+    }
+    const double *dl = dl32 ? dlw.data() : (const double *)pdl;
 
     std::vector<int32_t> l2g(nn);
     std::vector<char> seen(nn, 0);
@@ -819,13 +864,14 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     return MSOD_OK;
 }
 
-// Shared front half of the convective calls (TG: geometry, TF: fields).
-template <typename TG, typename TF>
+// Shared front half of the convective calls (TG: geometry, TF: fields, TO:
+// outputs; an FP32 derivative table comes with FP32 outputs).
+template <typename TG, typename TF, typename TO = double, typename TU = TF, typename TP = TF>
 int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const TG *He,
-                  const TG *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
-                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, const TF *u,
-                  const TF *q, const TF *rho, const TF *pr, const TF *E, double gamma_gas,
-                  double *Rmass, double *Rmom, double *Rener, int timer, bool fp32_arith = false)
+                  const TG *gpvol, const void *dlxigp_ip, const int32_t *invAtoIJK,
+                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, const TU *u,
+                  const TF *q, const TF *rho, const TP *pr, const TF *E, double gamma_gas,
+                  TO *Rmass, TO *Rmom, TO *Rener, int timer, bool fp32_arith = false)
 {
     NEED_INIT();
     int st = check_sizes(porder, nelem, npoin);
@@ -833,57 +879,58 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     const int qn = porder + 1;
     const int64_t nn = (int64_t)qn * qn * qn, NE = nelem, NP = npoin;
     TableSet *ts;
-    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts))) return st;
+    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts, sizeof(TO) == 4))) return st;
     const void *dc, *dHe, *dV, *du, *dq, *dr, *dp = nullptr, *dE, *dRm, *dRmo, *dRe;
     uint64_t vc, vHe, vV, vx;
     if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
     if ((st = resolve(He, sizeof(TG) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
     if ((st = resolve(gpvol, sizeof(TG) * NE * nn, "gpvol", &dV, &vV))) return st;
-    if ((st = resolve(u, sizeof(TF) * 3 * NP, "u", &du, &vx))) return st;
+    if ((st = resolve(u, sizeof(TU) * 3 * NP, "u", &du, &vx))) return st;
     if ((st = resolve(q, sizeof(TF) * 3 * NP, "q", &dq, &vx))) return st;
     if ((st = resolve(rho, sizeof(TF) * NP, "rho", &dr, &vx))) return st;
-    if (pr && (st = resolve(pr, sizeof(TF) * NP, "pr", &dp, &vx))) return st;
+    if (pr && (st = resolve(pr, sizeof(TP) * NP, "pr", &dp, &vx))) return st;
     if ((st = resolve(E, sizeof(TF) * NP, "E", &dE, &vx))) return st;
-    if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
-    if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
-    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
+    if ((st = resolve(Rmass, sizeof(TO) * NP, "Rmass", &dRm, &vx))) return st;
+    if ((st = resolve(Rmom, sizeof(TO) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(TO) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
     const TG *geo, *gvol;
     if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
     if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
-    ConvecArgs<TG, TF> a;
+    ConvecArgs<TG, TF, TO, TU, TP> a;
     a.nelem = NE;
     a.npoin = NP;
     a.geo = geo;
     a.gvol = gvol;
-    a.u = (const TF *)du;
+    a.u = (const TU *)du;
     a.q = (const TF *)dq;
     a.rho = (const TF *)dr;
-    a.pr = (const TF *)dp;
+    a.pr = (const TP *)dp;
     a.E = (const TF *)dE;
     a.gamma_m1 = gamma_gas - 1.0;
-    a.Rmass = (double *)dRm;
-    a.Rmom = (double *)dRmo;
-    a.Rener = (double *)dRe;
+    a.Rmass = (TO *)dRm;
+    a.Rmom = (TO *)dRmo;
+    a.Rener = (TO *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
     if ((st = run_assembled(a, *plan, timer,
-                            [&](const ConvecArgs<TG, TF> &x) {
-                                if constexpr (std::is_same<TG, float>::value && std::is_same<TF, float>::value)
+                            [&](const ConvecArgs<TG, TF, TO, TU, TP> &x) {
+                                if constexpr (std::is_same<ConvecArgs<TG, TF, TO, TU, TP>,
+                                                           ConvecArgs<float, float>>::value)
                                     if (fp32_arith) return launch_convec_fp32(porder, x, g.stream);
-                                return launch_convec<TG, TF>(porder, x, g.stream);
+                                return launch_convec(porder, x, g.stream);
                             },
                             "convection kernel launch", kConvecNatural)))
         return st;
     return finish_call("convection kernel");
 }
 
-template <typename TG, typename TF>
+template <typename TG, typename TF, typename TO = double, typename TU = TF>
 int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const TG *He,
-                 const TG *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                 const TG *gpvol, const void *dlxigp_ip, const int32_t *invAtoIJK,
                  const int32_t *AtoI, const int32_t *AtoJ, const int32_t *AtoK, double Cp,
-                 double Pr, const TF *rho, const TF *u, const TF *Tem, const TF *mu_fluid,
-                 const TG *mu_e, const TG *mu_sgs, double *Rmass, double *Rmom, double *Rener,
+                 double Pr, const TF *rho, const TU *u, const TF *Tem, const TF *mu_fluid,
+                 const TG *mu_e, const TG *mu_sgs, TO *Rmass, TO *Rmom, TO *Rener,
                  int timer, bool closure = false, const float *dmlp = nullptr, bool fp32_arith = false)
 {
     NEED_INIT();
@@ -892,33 +939,33 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     const int qn = porder + 1;
     const int64_t nn = (int64_t)qn * qn * qn, NE = nelem, NP = npoin;
     TableSet *ts;
-    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts))) return st;
+    if ((st = get_tables(porder, invAtoIJK, AtoI, AtoJ, AtoK, dlxigp_ip, &ts, sizeof(TO) == 4))) return st;
     const void *dc, *dHe, *dV, *dr, *du, *dT, *dmf, *dme, *dms = nullptr, *dRm, *dRmo, *dRe;
     uint64_t vc, vHe, vV, vx;
     if ((st = resolve(connec, sizeof(int32_t) * NE * nn, "connec", &dc, &vc))) return st;
     if ((st = resolve(He, sizeof(TG) * 9 * NE * nn, "He", &dHe, &vHe))) return st;
     if ((st = resolve(gpvol, sizeof(TG) * NE * nn, "gpvol", &dV, &vV))) return st;
     if ((st = resolve(rho, sizeof(TF) * NP, "rho", &dr, &vx))) return st;
-    if ((st = resolve(u, sizeof(TF) * 3 * NP, "u", &du, &vx))) return st;
+    if ((st = resolve(u, sizeof(TU) * 3 * NP, "u", &du, &vx))) return st;
     if ((st = resolve(Tem, sizeof(TF) * NP, "Tem", &dT, &vx))) return st;
     if ((st = resolve(mu_fluid, sizeof(TF) * NP, "mu_fluid", &dmf, &vx))) return st;
     if ((st = resolve(mu_e, sizeof(TG) * NE * nn, "mu_e", &dme, &vx))) return st;
     if (!closure && (st = resolve(mu_sgs, sizeof(TG) * NE * nn, "mu_sgs", &dms, &vx))) return st;
-    if ((st = resolve(Rmass, sizeof(double) * NP, "Rmass", &dRm, &vx))) return st;
-    if ((st = resolve(Rmom, sizeof(double) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
-    if ((st = resolve(Rener, sizeof(double) * NP, "Rener", &dRe, &vx))) return st;
+    if ((st = resolve(Rmass, sizeof(TO) * NP, "Rmass", &dRm, &vx))) return st;
+    if ((st = resolve(Rmom, sizeof(TO) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
+    if ((st = resolve(Rener, sizeof(TO) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
     const TG *geo, *gvol;
     if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
     if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
-    DiffuArgs<TG, TF> a;
+    DiffuArgs<TG, TF, TO, TU> a;
     a.nelem = NE;
     a.npoin = NP;
     a.geo = geo;
     a.gvol = gvol;
     a.rho = (const TF *)dr;
-    a.u = (const TF *)du;
+    a.u = (const TU *)du;
     a.Tem = (const TF *)dT;
     a.mu_fluid = (const TF *)dmf;
     a.mu_e = (const TG *)dme;
@@ -927,17 +974,18 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     a.Pr = Pr;
     a.c_rho = g.c_rho;
     a.c_ener = g.c_ener;
-    a.Rmass = (double *)dRm;
-    a.Rmom = (double *)dRmo;
-    a.Rener = (double *)dRe;
+    a.Rmass = (TO *)dRm;
+    a.Rmom = (TO *)dRmo;
+    a.Rener = (TO *)dRe;
     a.tab = Tables{ts->dD, ts->dl2g};
     if ((st = run_assembled(a, *plan, timer,
-                            [&](const DiffuArgs<TG, TF> &x) {
-                                if constexpr (std::is_same<TG, double>::value && std::is_same<TF, double>::value)
+                            [&](const DiffuArgs<TG, TF, TO, TU> &x) {
+                                using AT = DiffuArgs<TG, TF, TO, TU>;
+                                if constexpr (std::is_same<AT, DiffuArgs<double, double>>::value)
                                     if (dmlp) return launch_diffu_closure(porder, x, dmlp, g.stream);
-                                if constexpr (std::is_same<TG, float>::value && std::is_same<TF, float>::value)
+                                if constexpr (std::is_same<AT, DiffuArgs<float, float>>::value)
                                     if (fp32_arith) return launch_diffu_fp32(porder, x, g.stream);
-                                return launch_diffu<TG, TF>(porder, x, g.stream);
+                                return launch_diffu(porder, x, g.stream);
                             },
                             "diffusion kernel launch", kDiffuNatural)))
         return st;
@@ -1855,7 +1903,7 @@ int msod_full_convec_ijk_mixed(int porder, int nelem, int npoin, const int32_t *
     MsodRange rg_("msod_full_convec_ijk_mixed");
     if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
     return convec_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
-                                gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, (const float *)nullptr, E, gamma_gas,
                                 Rmass, Rmom, Rener, 3);
 }
 
@@ -1891,6 +1939,66 @@ int msod_full_diffusion_ijk_mixed(int porder, int nelem, int npoin, const int32_
                                mu_sgs, Rmass, Rmom, Rener, 4);
 }
 
+int msod_full_convec_ijk_f32split(int porder, int nelem, int npoin, const int32_t *connec, const float *He,
+                                  const float *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                                  const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                                  const double *u, const float *q, const float *rho, const double *pr,
+                                  const float *E, double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_convec_ijk_f32split");
+    if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
+    return convec_common<float, float, double, double, double>(
+        porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho,
+        pr, E, 0.0, Rmass, Rmom, Rener, 18);
+}
+
+int msod_full_diffusion_ijk_f32split(int porder, int nelem, int npoin, const int32_t *connec, const float *He,
+                                     const float *gpvol, const double *dlxigp_ip, const int32_t *invAtoIJK,
+                                     const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                                     double Cp, double Pr, const float *rho, const double *u, const float *Tem,
+                                     const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                                     double *Rmass, double *Rmom, double *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk_f32split");
+    return diffu_common<float, float, double, double>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip,
+                                                      invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u,
+                                                      Tem, mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener, 19);
+}
+
+int msod_full_convec_ijk_r4(int porder, int nelem, int npoin, const int32_t *connec, const float *Ngp,
+                            const float *dNgp, const float *He, const float *gpvol,
+                            const float *dlxigp_ip, const float *xgp, const int32_t *invAtoIJK,
+                            const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                            const float *u, const float *q, const float *rho, const float *pr,
+                            const float *E, float *Rmass, float *Rmom, float *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_convec_ijk_r4");
+    (void)Ngp; (void)dNgp; (void)xgp;
+    if (!pr) return fail(MSOD_ERR_ARG, "pr is NULL");
+    return convec_common<float, float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                              gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, 0.0, Rmass,
+                                              Rmom, Rener, 16);
+}
+
+int msod_full_diffusion_ijk_r4(int porder, int nelem, int npoin, const int32_t *connec, const float *Ngp,
+                               const float *dNgp, const float *He, const float *gpvol,
+                               const float *dlxigp_ip, const float *xgp, const int32_t *invAtoIJK,
+                               const int32_t *gmshAtoI, const int32_t *gmshAtoJ, const int32_t *gmshAtoK,
+                               float Cp, float Pr, const float *rho, const float *u, const float *Tem,
+                               const float *mu_fluid, const float *mu_e, const float *mu_sgs,
+                               float *Rmass, float *Rmom, float *Rener)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    MsodRange rg_("msod_full_diffusion_ijk_r4");
+    (void)Ngp; (void)dNgp; (void)xgp;
+    return diffu_common<float, float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
+                                             gmshAtoI, gmshAtoJ, gmshAtoK, (double)Cp, (double)Pr, rho, u, Tem,
+                                             mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener, 17);
+}
+
 int msod_full_convec_ijk_fp32(int porder, int nelem, int npoin, const int32_t *connec,
                               const float *He, const float *gpvol, const double *dlxigp_ip,
                               const int32_t *invAtoIJK, const int32_t *gmshAtoI,
@@ -1902,7 +2010,7 @@ int msod_full_convec_ijk_fp32(int porder, int nelem, int npoin, const int32_t *c
     MsodRange rg_("msod_full_convec_ijk_fp32");
     if (!(gamma_gas > 1.0)) return fail(MSOD_ERR_ARG, "gamma_gas=%g must exceed 1", gamma_gas);
     return convec_common<float, float>(porder, nelem, npoin, connec, He, gpvol, dlxigp_ip, invAtoIJK,
-                                gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, nullptr, E, gamma_gas,
+                                gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, (const float *)nullptr, E, gamma_gas,
                                 Rmass, Rmom, Rener, 14, true);
 }
 

@@ -9,7 +9,10 @@
 !     mod_constants (compile time); the dummies are assumed-shape, and every
 !     array must be contiguous and `present` on the device (msod_copyin /
 !     msod_create of module msod_device, the `enter data` counterparts).
-!   * real kind 8 (the reference's rp is a build option, default 4).
+!   * full_convec_ijk and full_diffusion_ijk are generic: a caller's real(8)
+!     arrays reach the FP64 kernels, real(4) arrays (the reference built at its
+!     default rp = 4, mod_constants.f90:5) the float drop-in, which computes in
+!     FP64 and returns real(4) residuals.  The other procedures take real(8).
 !   * c_rho and c_ener, module globals of mod_numerical_params in the
 !     reference (mod_numerical_params.f90:26-27), are module variables of
 !     elem_diffu here, with the same defaults.
@@ -47,16 +50,21 @@ end module msod_tet_shapes
 module elem_convec
 
    use iso_c_binding
-   use msod_device, only: msod_full_convec_ijk, msod_fem_convec, msod_check
+   use msod_device, only: msod_full_convec_ijk, msod_full_convec_ijk_r4, msod_fem_convec, msod_check
    use msod_tet_shapes, only: check_tet_shapes
    implicit none
    private
    public :: full_convec_ijk, fem_convec
 
+   ! elem_convec.f90:198-211 at either real kind of the reference
+   interface full_convec_ijk
+      module procedure full_convec_ijk_r8, full_convec_ijk_r4
+   end interface full_convec_ijk
+
 contains
 
-   subroutine full_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
-                              gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener)
+   subroutine full_convec_ijk_r8(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener)
       integer(4), intent(in) :: nelem, npoin
       integer(4), intent(in), contiguous :: connec(:,:)
       real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
@@ -71,7 +79,32 @@ contains
       call msod_check(msod_full_convec_ijk(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, He, &
                       gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, &
                       Rmass, Rmom, Rener), 'full_convec_ijk')
-   end subroutine full_convec_ijk
+   end subroutine full_convec_ijk_r8
+
+   ! The same at rp = 4: real(4) arrays and residuals.
+   subroutine full_convec_ijk_r4(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                                 gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(4), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
+      real(4), intent(in), contiguous :: gpvol(:,:,:)
+      integer(4), intent(in), contiguous :: invAtoIJK(:,:,:), gmshAtoI(:), gmshAtoJ(:), gmshAtoK(:)
+      real(4), intent(in), contiguous :: q(:,:), u(:,:), rho(:), pr(:), E(:)
+      real(4), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      integer :: porder, nn
+      porder = size(invAtoIJK, 1) - 1
+      nn = (porder+1)**3
+      if (size(connec,1) /= nelem .or. size(connec,2) /= nn .or. size(He,3) /= nn .or. &
+          size(He,4) < nelem .or. size(gpvol,2) /= nn .or. size(gpvol,3) < nelem .or. &
+          size(u,1) < npoin .or. size(Rmass) < npoin .or. size(Rmom,1) < npoin .or. size(Rener) < npoin) then
+         write(*,'(a,i0,a,i0,a,i0)') 'msod: full_convec_ijk (real(4)): array shapes do not match nelem=', &
+              nelem, ' npoin=', npoin, ' porder=', porder
+         error stop 2
+      end if
+      call msod_check(msod_full_convec_ijk_r4(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, He, &
+                      gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, u, q, rho, pr, E, &
+                      Rmass, Rmom, Rener), 'full_convec_ijk (real(4))')
+   end subroutine full_convec_ijk_r4
 
    subroutine check_shapes(who, nelem, npoin, porder, connec, He, gpvol, nu, nr1, nr2, nr3)
       character(*), intent(in) :: who
@@ -113,7 +146,7 @@ module elem_diffu
 
    use iso_c_binding
    use msod_device, only: msod_full_diffusion_ijk, msod_set_diffusion_coefficients, msod_check, &
-                          msod_full_diffusion_ijk_closure, msod_fem_diffu
+                          msod_full_diffusion_ijk_closure, msod_fem_diffu, msod_full_diffusion_ijk_r4
    use msod_tet_shapes, only: check_tet_shapes
    implicit none
    private
@@ -122,11 +155,16 @@ module elem_diffu
    real(8) :: c_rho = 1.0d0
    real(8) :: c_ener = 1.0d0
 
+   ! elem_diffu.f90:10-23 at either real kind of the reference
+   interface full_diffusion_ijk
+      module procedure full_diffusion_ijk_r8, full_diffusion_ijk_r4
+   end interface full_diffusion_ijk
+
 contains
 
-   subroutine full_diffusion_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
-                                 gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
-                                 Rmass, Rmom, Rener)
+   subroutine full_diffusion_ijk_r8(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                                    gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
+                                    Rmass, Rmom, Rener)
       integer(4), intent(in) :: nelem, npoin
       integer(4), intent(in), contiguous :: connec(:,:)
       real(8), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
@@ -149,7 +187,35 @@ contains
       call msod_check(msod_full_diffusion_ijk(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, He, &
                       gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, &
                       mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener), 'full_diffusion_ijk')
-   end subroutine full_diffusion_ijk
+   end subroutine full_diffusion_ijk_r8
+
+   ! The same at rp = 4: real(4) arrays, Cp, Pr and residuals.
+   subroutine full_diffusion_ijk_r4(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
+                                    gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
+                                    Rmass, Rmom, Rener)
+      integer(4), intent(in) :: nelem, npoin
+      integer(4), intent(in), contiguous :: connec(:,:)
+      real(4), intent(in), contiguous :: Ngp(:,:), dNgp(:,:,:), He(:,:,:,:), xgp(:,:), dlxigp_ip(:,:,:)
+      real(4), intent(in), contiguous :: gpvol(:,:,:)
+      integer(4), intent(in), contiguous :: invAtoIJK(:,:,:), gmshAtoI(:), gmshAtoJ(:), gmshAtoK(:)
+      real(4), intent(in) :: Cp, Pr
+      real(4), intent(in), contiguous :: rho(:), u(:,:), Tem(:), mu_fluid(:), mu_e(:,:), mu_sgs(:,:)
+      real(4), intent(out), contiguous :: Rmass(:), Rmom(:,:), Rener(:)
+      integer :: porder, nn
+      porder = size(invAtoIJK, 1) - 1
+      nn = (porder+1)**3
+      if (size(connec,1) /= nelem .or. size(connec,2) /= nn .or. size(He,4) < nelem .or. &
+          size(mu_e,1) /= nelem .or. size(mu_e,2) /= nn .or. size(mu_sgs,1) /= nelem .or. &
+          size(u,1) < npoin .or. size(Rmass) < npoin .or. size(Rmom,1) < npoin) then
+         write(*,'(a,i0,a,i0,a,i0)') 'msod: full_diffusion_ijk (real(4)): array shapes do not match nelem=', &
+              nelem, ' npoin=', npoin, ' porder=', porder
+         error stop 2
+      end if
+      call msod_check(msod_set_diffusion_coefficients(c_rho, c_ener), 'set_diffusion_coefficients')
+      call msod_check(msod_full_diffusion_ijk_r4(int(porder, c_int), nelem, npoin, connec, Ngp, dNgp, He, &
+                      gpvol, dlxigp_ip, xgp, invAtoIJK, gmshAtoI, gmshAtoJ, gmshAtoK, Cp, Pr, rho, u, Tem, &
+                      mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener), 'full_diffusion_ijk (real(4))')
+   end subroutine full_diffusion_ijk_r4
 
    ! full_diffusion_ijk with mu_sgs predicted by the learned closure inside the
    ! kernel (no counterpart in the reference).  mlp_params: the 1530 real(4)

@@ -21,6 +21,7 @@ module msod_device
    public :: msod_full_diffusion_ijk_mixed, msod_set_diffusion_coefficients
    public :: msod_full_diffusion_ijk_closure, msod_jacobian_device
    public :: msod_tet_jacobian_device, msod_fem_convec, msod_fem_diffu
+   public :: msod_full_convec_ijk_r4, msod_full_diffusion_ijk_r4
 
    interface
       integer(c_int) function msod_init(device) bind(C, name='msod_init')
@@ -95,6 +96,29 @@ module msod_device
          type(*), dimension(*), intent(in) :: rho, u, Tem, mu_fluid, mu_e, mu_sgs
          type(*), dimension(*) :: Rmass, Rmom, Rener
       end function msod_full_diffusion_ijk
+
+      ! the drop-in at the reference's default real kind (rp = 4)
+      integer(c_int) function msod_full_convec_ijk_r4(porder, nelem, npoin, connec, Ngp, dNgp, He, gpvol, &
+            dlxigp_ip, xgp, invAtoIJK, AtoI, AtoJ, AtoK, u, q, rho, pr, E, Rmass, Rmom, Rener) &
+            bind(C, name='msod_full_convec_ijk_r4')
+         import :: c_int
+         integer(c_int), value :: porder, nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp
+         type(*), dimension(*), intent(in) :: invAtoIJK, AtoI, AtoJ, AtoK, u, q, rho, pr, E
+         type(*), dimension(*) :: Rmass, Rmom, Rener
+      end function msod_full_convec_ijk_r4
+
+      integer(c_int) function msod_full_diffusion_ijk_r4(porder, nelem, npoin, connec, Ngp, dNgp, He, gpvol, &
+            dlxigp_ip, xgp, invAtoIJK, AtoI, AtoJ, AtoK, Cp, Pr, rho, u, Tem, mu_fluid, mu_e, mu_sgs, &
+            Rmass, Rmom, Rener) bind(C, name='msod_full_diffusion_ijk_r4')
+         import :: c_int, c_float
+         integer(c_int), value :: porder, nelem, npoin
+         type(*), dimension(*), intent(in) :: connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp
+         type(*), dimension(*), intent(in) :: invAtoIJK, AtoI, AtoJ, AtoK
+         real(c_float), value :: Cp, Pr
+         type(*), dimension(*), intent(in) :: rho, u, Tem, mu_fluid, mu_e, mu_sgs
+         type(*), dimension(*) :: Rmass, Rmom, Rener
+      end function msod_full_diffusion_ijk_r4
 
       integer(c_int) function msod_full_convec_ijk_mixed(porder, nelem, npoin, connec, He, gpvol, &
             dlxigp_ip, invAtoIJK, AtoI, AtoJ, AtoK, u, q, rho, E, gamma_gas, Rmass, Rmom, Rener) &

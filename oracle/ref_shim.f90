@@ -7,6 +7,12 @@
 ! reference driver (main.f90:131-212), which lives in main.f90 -- a file that
 ! cannot be compiled here because it needs MPI -- and is restated below.
 !
+! The real arrays of the kernel and geometry wrappers have the reference's
+! own kind real(rp): double in the rp=8 libraries (oracle/_ref/libsodref_pP.so),
+! float in the rp=4 ones (oracle/_ref/r4/libsodref_r4_pP.so, the reference's
+! default kind, mod_constants.f90:5).  The tables are returned in double
+! (exact: they are computed in real(rp)).
+!
 ! Every entry point is a thin C-callable wrapper:
 !   sodref_porder / sodref_rp        compile-time porder and real kind
 !   sodref_tables                    main.f90:110-255 (tables + index maps)
@@ -128,9 +134,9 @@ contains
    ! The reference's own disjoint mesh: mod_mesh.f90:11-65.
    subroutine sodref_create_mesh(nelem, npoin, xyzBase, connec, xyz) bind(C, name='sodref_create_mesh')
       integer(c_int), value :: nelem, npoin
-      real(c_double), intent(in) :: xyzBase(nnode,ndime)
+      real(rp), intent(in) :: xyzBase(nnode,ndime)
       integer(c_int), intent(out) :: connec(nelem,nnode)
-      real(c_double), intent(out) :: xyz(npoin,ndime)
+      real(rp), intent(out) :: xyz(npoin,ndime)
       call create_mesh(nelem, npoin, xyzBase, connec, xyz)
    end subroutine sodref_create_mesh
 
@@ -138,8 +144,8 @@ contains
          bind(C, name='sodref_elem_jacobian')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,nnode)
-      real(c_double), intent(in) :: coord(npoin,ndime), dNgp(ndime,nnode,ngaus), wgp(ngaus)
-      real(c_double), intent(out) :: gpvol(1,ngaus,nelem), He(ndime,ndime,ngaus,nelem)
+      real(rp), intent(in) :: coord(npoin,ndime), dNgp(ndime,nnode,ngaus), wgp(ngaus)
+      real(rp), intent(out) :: gpvol(1,ngaus,nelem), He(ndime,ndime,ngaus,nelem)
       call elem_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He)
    end subroutine sodref_elem_jacobian
 
@@ -148,12 +154,12 @@ contains
          bind(C, name='sodref_full_convec_ijk')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,nnode)
-      real(c_double), intent(in) :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
-      real(c_double), intent(in) :: He(ndime,ndime,ngaus,nelem), xgp(ngaus,ndime), dlxigp_ip(ngaus,ndime,porder+1)
-      real(c_double), intent(in) :: gpvol(1,ngaus,nelem)
+      real(rp), intent(in) :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
+      real(rp), intent(in) :: He(ndime,ndime,ngaus,nelem), xgp(ngaus,ndime), dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp), intent(in) :: gpvol(1,ngaus,nelem)
       integer(c_int), intent(in) :: invAtoIJK(porder+1,porder+1,porder+1), AtoI(nnode), AtoJ(nnode), AtoK(nnode)
-      real(c_double), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
-      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
+      real(rp), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
       call full_convec_ijk(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
                            AtoI, AtoJ, AtoK, u, q, rho, pr, E, Rmass, Rmom, Rener)
    end subroutine sodref_full_convec_ijk
@@ -163,12 +169,12 @@ contains
          bind(C, name='sodref_full_convec_ijk_split')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,nnode)
-      real(c_double), intent(in) :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
-      real(c_double), intent(in) :: He(ndime,ndime,ngaus,nelem), xgp(ngaus,ndime), dlxigp_ip(ngaus,ndime,porder+1)
-      real(c_double), intent(in) :: gpvol(1,ngaus,nelem)
+      real(rp), intent(in) :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
+      real(rp), intent(in) :: He(ndime,ndime,ngaus,nelem), xgp(ngaus,ndime), dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp), intent(in) :: gpvol(1,ngaus,nelem)
       integer(c_int), intent(in) :: invAtoIJK(porder+1,porder+1,porder+1), AtoI(nnode), AtoJ(nnode), AtoK(nnode)
-      real(c_double), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
-      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
+      real(rp), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
       call full_convec_ijk_split(nelem, npoin, connec, Ngp, dNgp, He, gpvol, dlxigp_ip, xgp, invAtoIJK, &
                                  AtoI, AtoJ, AtoK, u, q, rho, pr, E, Rmass, Rmom, Rener)
    end subroutine sodref_full_convec_ijk_split
@@ -180,14 +186,14 @@ contains
          Rmass, Rmom, Rener) bind(C, name='sodref_full_diffusion_ijk')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,nnode)
-      real(c_double), intent(in) :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
-      real(c_double), intent(in) :: He(ndime,ndime,ngaus,nelem), xgp(ngaus,ndime), dlxigp_ip(ngaus,ndime,porder+1)
-      real(c_double), intent(in) :: gpvol(1,ngaus,nelem)
+      real(rp), intent(in) :: Ngp(ngaus,nnode), dNgp(ndime,nnode,ngaus)
+      real(rp), intent(in) :: He(ndime,ndime,ngaus,nelem), xgp(ngaus,ndime), dlxigp_ip(ngaus,ndime,porder+1)
+      real(rp), intent(in) :: gpvol(1,ngaus,nelem)
       integer(c_int), intent(in) :: invAtoIJK(porder+1,porder+1,porder+1), AtoI(nnode), AtoJ(nnode), AtoK(nnode)
       real(c_double), value :: Cp, Pr, c_rho_in, c_ener_in
-      real(c_double), intent(in) :: rho(npoin), u(npoin,ndime), Tem(npoin), mu_fluid(npoin)
-      real(c_double), intent(in) :: mu_e(nelem,ngaus), mu_sgs(nelem,ngaus)
-      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp), intent(in) :: rho(npoin), u(npoin,ndime), Tem(npoin), mu_fluid(npoin)
+      real(rp), intent(in) :: mu_e(nelem,ngaus), mu_sgs(nelem,ngaus)
+      real(rp), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
       real(rp) :: Cp_r, Pr_r
       c_rho = c_rho_in
       c_ener = c_ener_in
@@ -215,9 +221,9 @@ contains
 
    subroutine sodref_gen_tet_mesh(nelem, npoin, xyzBase, connec, xyz) bind(C, name='sodref_gen_tet_mesh')
       integer(c_int), value :: nelem, npoin
-      real(c_double), intent(in) :: xyzBase(4,ndime)
+      real(rp), intent(in) :: xyzBase(4,ndime)
       integer(c_int), intent(out) :: connec(nelem,4)
-      real(c_double), intent(out) :: xyz(npoin,ndime)
+      real(rp), intent(out) :: xyz(npoin,ndime)
       call gen_tet_mesh(nelem, npoin, xyzBase, connec, xyz)
    end subroutine sodref_gen_tet_mesh
 
@@ -225,8 +231,8 @@ contains
          bind(C, name='sodref_tet_jacobian')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,4)
-      real(c_double), intent(in) :: coord(npoin,ndime), dNgp(ndime,4,4), wgp(4)
-      real(c_double), intent(out) :: gpvol(1,4,nelem), He(ndime,ndime,4,nelem)
+      real(rp), intent(in) :: coord(npoin,ndime), dNgp(ndime,4,4), wgp(4)
+      real(rp), intent(out) :: gpvol(1,4,nelem), He(ndime,ndime,4,nelem)
       call tet_jacobian(nelem, npoin, connec, coord, dNgp, wgp, gpvol, He)
    end subroutine sodref_tet_jacobian
 
@@ -234,9 +240,9 @@ contains
          Rmass, Rmom, Rener) bind(C, name='sodref_fem_convec')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,4)
-      real(c_double), intent(in) :: Ngp(4,4), dNgp(ndime,4,4), He(ndime,ndime,4,nelem), gpvol(1,4,nelem)
-      real(c_double), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
-      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp), intent(in) :: Ngp(4,4), dNgp(ndime,4,4), He(ndime,ndime,4,nelem), gpvol(1,4,nelem)
+      real(rp), intent(in) :: q(npoin,ndime), u(npoin,ndime), rho(npoin), pr(npoin), E(npoin)
+      real(rp), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
       call fem_convec(nelem, npoin, connec, Ngp, dNgp, He, gpvol, u, q, rho, pr, E, Rmass, Rmom, Rener)
    end subroutine sodref_fem_convec
 
@@ -244,11 +250,11 @@ contains
          rho, u, Tem, mu_fluid, mu_e, mu_sgs, Rmass, Rmom, Rener) bind(C, name='sodref_fem_diffu')
       integer(c_int), value :: nelem, npoin
       integer(c_int), intent(in) :: connec(nelem,4)
-      real(c_double), intent(in) :: Ngp(4,4), dNgp(ndime,4,4), He(ndime,ndime,4,nelem), gpvol(1,4,nelem)
+      real(rp), intent(in) :: Ngp(4,4), dNgp(ndime,4,4), He(ndime,ndime,4,nelem), gpvol(1,4,nelem)
       real(c_double), value :: Cp, Pr, c_rho_in, c_ener_in
-      real(c_double), intent(in) :: rho(npoin), u(npoin,ndime), Tem(npoin), mu_fluid(npoin)
-      real(c_double), intent(in) :: mu_e(nelem,4), mu_sgs(nelem,4)
-      real(c_double), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
+      real(rp), intent(in) :: rho(npoin), u(npoin,ndime), Tem(npoin), mu_fluid(npoin)
+      real(rp), intent(in) :: mu_e(nelem,4), mu_sgs(nelem,4)
+      real(rp), intent(out) :: Rmass(npoin), Rmom(npoin,ndime), Rener(npoin)
       real(rp) :: Cp_r, Pr_r
       c_rho = c_rho_in
       c_ener = c_ener_in

@@ -24,8 +24,13 @@ tell a generator change from a kernel change.  Fixtures:
                   commit whole (64^3 p=4, 48^3 p=7, 96^3 p=3): per-component
                   L2 norms and sums, and the values at 4096 seeded nodes
 
+  r4_*.npz        the reference compiled at its default real kind rp=4
+                  (oracle/_ref/r4/): its own tables (float32 values) and its
+                  residuals for the case's inputs rounded to float32 (SHA-256
+                  of each rounded array kept)
+
 `python tests/golden/make_golden.py --tet` writes the TET fixtures only,
-`--digests` the digests only.
+`--digests` the digests only, `--r4` the rp=4 fixtures only.
 """
 import hashlib
 import json
@@ -40,7 +45,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
 import minisod2d_amd as m  # noqa: E402
-from oracle_lib import RefLib, RefTet  # noqa: E402
+from oracle_lib import R4_FIELDS, RefLib, RefLib4, RefTet, r4_inputs  # noqa: E402
 
 # (name, builder) -- small enough for the reference to finish in seconds
 CASES = {
@@ -152,7 +157,34 @@ def write_digests():
         print(name, prob.nelem, prob.npoin, "ok", flush=True)
 
 
+# (name, builder) -- cases of the rp=4 drop-in
+R4_CASES = {
+    "r4_box8_p3": lambda: m.box_problem(8, 3),
+    "r4_box3x2x2_p4_jit": lambda: m.box_problem(3, 4, ny=2, nz=2, jitter=0.05),
+    "r4_box2_p7_jit_jump": lambda: m.box_problem(2, 7, jitter=0.05, rho_jump=0.1),
+}
+
+
+def write_r4():
+    for name, build in R4_CASES.items():
+        prob = build()
+        ref = RefLib4(prob.porder)
+        t = ref.tables()
+        c = r4_inputs(prob, t)
+        Rmass, Rmom, Rener = ref.convec(c)
+        Dmass, Dmom, Dener = ref.diffu(c)
+        meta = dict(prob.meta, nelem=prob.nelem, npoin=prob.npoin, Cp=prob.Cp, Pr=prob.Pr,
+                    inputs={k: digest(c[k]) for k in R4_FIELDS + ("connec",)})
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), Rmass=Rmass, Rmom=Rmom, Rener=Rener,
+                            Dmass=Dmass, Dmom=Dmom, Dener=Dener, dlxigp_ip=t["dlxigp_ip"].astype(np.float32),
+                            xgp=t["xgp"].astype(np.float32), meta=json.dumps(meta))
+        print(name, prob.nelem, prob.npoin, "ok", flush=True)
+
+
 def main():
+    if "--r4" in sys.argv[1:]:
+        write_r4()
+        return
     if "--tet" in sys.argv[1:]:
         write_tet()
         return

@@ -202,6 +202,76 @@ class RefLib:
         return R
 
 
+class RefLib4:
+    """The reference's Fortran kernels at its default real kind rp=4
+    (mod_constants.f90:5): every real array is float32; the tables come back
+    in float64 holding float32 values."""
+
+    def __init__(self, p):
+        path = os.path.join(REF_DIR, "r4", f"libsodref_r4_p{p}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"reference library not built: {path}")
+        L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+        assert L.sodref_porder() == p and L.sodref_rp() == 4
+        L.sodref_full_convec_ijk.argtypes = [ctypes.c_int] * 2 + [_vp] * 19
+        L.sodref_full_convec_ijk_split.argtypes = [ctypes.c_int] * 2 + [_vp] * 19
+        L.sodref_full_diffusion_ijk.argtypes = [ctypes.c_int] * 2 + [_vp] * 11 + [_d] * 4 + [_vp] * 9
+        self.L = L
+        self.p = p
+
+    def tables(self):
+        return RefLib.tables(self)
+
+    @staticmethod
+    def _f32(a):
+        assert a.dtype == np.float32 and a.flags.f_contiguous, a.dtype
+        return _p(a)
+
+    def convec(self, c):
+        """c: dict of float32/int32 Fortran arrays (names of the reference's
+        arguments).  Rmass, Rmom of full_convec_ijk, Rener of the split kernel."""
+        n = c["npoin"]
+        out = lambda shape: np.zeros(shape, np.float32, order="F")
+        Rf = [out(n), out((n, 3)), out(n)]
+        Rs = [out(n), out((n, 3)), out(n)]
+        args = (c["nelem"], n, _p(c["connec"]), *map(self._f32, (c["Ngp"], c["dNgp"], c["He"], c["gpvol"],
+                                                                  c["dlxigp_ip"], c["xgp"])),
+                _p(c["invAtoIJK"]), _p(c["AtoI"]), _p(c["AtoJ"]), _p(c["AtoK"]),
+                *map(self._f32, (c["u"], c["q"], c["rho"], c["pr"], c["E"])))
+        self.L.sodref_full_convec_ijk(*args, *map(_p, Rf))
+        self.L.sodref_full_convec_ijk_split(*args, *map(_p, Rs))
+        return Rf[0], Rf[1], Rs[2]
+
+    def diffu(self, c, c_rho=1.0, c_ener=1.0):
+        n = c["npoin"]
+        R = [np.zeros(n, np.float32, order="F"), np.zeros((n, 3), np.float32, order="F"),
+             np.zeros(n, np.float32, order="F")]
+        self.L.sodref_full_diffusion_ijk(
+            c["nelem"], n, _p(c["connec"]), *map(self._f32, (c["Ngp"], c["dNgp"], c["He"], c["gpvol"],
+                                                             c["dlxigp_ip"], c["xgp"])),
+            _p(c["invAtoIJK"]), _p(c["AtoI"]), _p(c["AtoJ"]), _p(c["AtoK"]), c["Cp"], c["Pr"], c_rho, c_ener,
+            *map(self._f32, (c["rho"], c["u"], c["Tem"], c["mu_fluid"], c["mu_e"], c["mu_sgs"])), *map(_p, R))
+        return R
+
+
+R4_FIELDS = ("He", "gpvol", "u", "q", "rho", "pr", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")
+R4_TABLES = ("Ngp", "dNgp", "dlxigp_ip", "xgp")
+
+
+def r4_inputs(prob, tables):
+    """The arguments a caller built at rp=4 passes: the problem's arrays
+    rounded to float32, the rp=4 reference's own tables (float32 values)."""
+    f32 = lambda a: np.asfortranarray(a, dtype=np.float32)
+    c = dict(nelem=prob.nelem, npoin=prob.npoin, Cp=prob.Cp, Pr=prob.Pr, connec=prob.connec)
+    for k in R4_FIELDS:
+        c[k] = f32(getattr(prob, k))
+    for k in R4_TABLES:
+        c[k] = f32(tables[k])
+    for k in ("invAtoIJK", "AtoI", "AtoJ", "AtoK"):
+        c[k] = np.asfortranarray(tables[k], dtype=np.int32)
+    return c
+
+
 class RefTet:
     """The reference's TET04 routines (any porder's library holds them)."""
 

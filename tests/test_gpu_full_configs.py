@@ -134,3 +134,137 @@ def test_full_size_sampled_parity_and_uniform_state(gpu, n, p, jitter, jump):
     for key, r, s in zip(KEYS, flat, scale):
         rms = float(np.sqrt(np.mean(np.square(r))))
         assert rms <= 1e-10 * s, (key, rms, s)
+
+
+# ---------------------------------------------------------------------------
+# precision forms at full size
+# ---------------------------------------------------------------------------
+SPLIT_F32 = ("He", "gpvol", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")   # u, pr stay FP64
+ALL_F32 = SPLIT_F32 + ("u", "pr")
+BAR = 1e-6   # north-star bar of the FP32 split against the FP64 oracle
+
+
+def rounded(prob, fields):
+    """A copy of prob with `fields` rounded to FP32 (widened back to FP64)."""
+    import copy
+    q = copy.copy(prob)
+    for k in fields:
+        setattr(q, k, np.asfortranarray(getattr(prob, k).astype(np.float32).astype(np.float64)))
+    return q
+
+
+def run_form(prob, form):
+    t = prob.tables
+    f = prob.fp32()
+    R, D = prob.outputs(), prob.outputs()
+    if form == "f32split":
+        ins = (prob.connec, f["He"], f["gpvol"], prob.u, f["q"], f["rho"], prob.pr, f["E"], f["Tem"],
+               f["mu_fluid"], f["mu_e"], f["mu_sgs"])
+    else:
+        ins = (prob.connec, f["He"], f["gpvol"], f["u"], f["q"], f["rho"], f["E"], f["Tem"], f["mu_fluid"],
+               f["mu_e"], f["mu_sgs"])
+    m.copyin(*ins)
+    m.create(*R, *D)
+    try:
+        if form == "f32split":
+            m.full_convec_ijk_f32split(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip,
+                                       t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.u, f["q"], f["rho"], prob.pr,
+                                       f["E"], *R)
+            m.full_diffusion_ijk_f32split(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"],
+                                          t.dlxigp_ip, t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp, prob.Pr,
+                                          f["rho"], prob.u, f["Tem"], f["mu_fluid"], f["mu_e"], f["mu_sgs"], *D)
+        else:
+            conv = m.full_convec_ijk_fp32 if form == "fp32" else m.full_convec_ijk_mixed
+            diff = m.full_diffusion_ijk_fp32 if form == "fp32" else m.full_diffusion_ijk_mixed
+            conv(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip, t.invAtoIJK, t.AtoI,
+                 t.AtoJ, t.AtoK, f["u"], f["q"], f["rho"], f["E"], prob.gamma, *R)
+            diff(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip, t.invAtoIJK, t.AtoI,
+                 t.AtoJ, t.AtoK, prob.Cp, prob.Pr, f["rho"], f["u"], f["Tem"], f["mu_fluid"], f["mu_e"],
+                 f["mu_sgs"], *D)
+        m.update_host(*R, *D)
+    finally:
+        m.delete(*ins, *R, *D)
+    return R + D
+
+
+@pytest.fixture(scope="module", params=[(128, 4, 0.0, 0.0), (48, 7, 0.05, 0.1)],
+                ids=["128cubed_p4", "48cubed_p7_jump"])
+def precision_case(request):
+    n, p, jitter, jump = request.param
+    say(f"[gpu] precision forms {n}^3 p={p}: building the mesh")
+    prob = m.box_problem(n, p, jitter=jitter, rho_jump=jump, threads=THREADS)
+    nodes = np.random.default_rng(23 + p).choice(prob.npoin, size=2000, replace=False) + 1
+    say(f"[gpu] precision forms {n}^3 p={p}: FP64 oracle at 2000 nodes")
+    want = sampled_oracle(prob, nodes)
+    out = {}
+    for form in ("f32split", "mixed", "fp32"):
+        say(f"[gpu] precision forms {n}^3 p={p}: {form} kernels")
+        out[form] = [r[nodes - 1] for r in run_form(prob, form)]
+    yield dict(n=n, p=p, prob=prob, nodes=nodes, want=want, out=out)
+
+
+def test_f32split_within_bar_at_full_size(gpu, precision_case):
+    """The FP32 split (FP64 u and pr) against the FP64 oracle on the FP64
+    inputs: every residual within BAR = 1e-6; against the oracle on the same
+    rounded inputs, 1e-12 (the only error is the storage of the inputs)."""
+    c = precision_case
+    got = c["out"]["f32split"]
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, got, c["want"])}
+    say(f"    f32split {c['n']}^3 p={c['p']} vs FP64 oracle: " + " ".join(f"{k}={v:.2e}" for k, v in e2e.items()))
+    want_r = sampled_oracle(rounded(c["prob"], SPLIT_F32), c["nodes"])
+    for k, g, w in zip(KEYS, got, want_r):
+        assert rel_l2(g, w) <= 1e-12, (k, rel_l2(g, w))
+    for k, v in e2e.items():
+        assert v <= BAR, (k, v)
+
+
+def test_mixed_equals_oracle_on_its_rounded_inputs_at_full_size(gpu, precision_case):
+    """Every input in FP32, FP64 arithmetic: equal to the FP64 oracle on the
+    same rounded inputs (1e-12); the distance to the FP64 inputs' oracle is
+    printed (it is the input rounding, see the xfail below)."""
+    c = precision_case
+    got = c["out"]["mixed"]
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, got, c["want"])}
+    say(f"    mixed {c['n']}^3 p={c['p']} vs FP64 oracle: " + " ".join(f"{k}={v:.2e}" for k, v in e2e.items()))
+    q = rounded(c["prob"], ALL_F32)
+    q.pr = None   # the mixed form recovers the pressure from the rounded state
+    want_r = sampled_oracle_recovered_p(q, c["nodes"])
+    for k, g, w in zip(KEYS, got, want_r):
+        assert rel_l2(g, w) <= 1e-12, (k, rel_l2(g, w))
+
+
+def test_fp32_arithmetic_error_at_full_size(gpu, precision_case):
+    """FP32 gathers, fluxes and sweeps: against the FP64 oracle on the same
+    rounded inputs the FP32 arithmetic adds at most 5e-5 (printed per
+    residual; measured 2.7e-5 in the diffusive momentum residual at both
+    sizes, whose two derivative sweeps cancel, and <= 4e-6 elsewhere)."""
+    c = precision_case
+    got = c["out"]["fp32"]
+    q = rounded(c["prob"], ALL_F32)
+    q.pr = None
+    want_r = sampled_oracle_recovered_p(q, c["nodes"])
+    kern = {k: rel_l2(g, w) for k, g, w in zip(KEYS, got, want_r)}
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, got, c["want"])}
+    say(f"    fp32 {c['n']}^3 p={c['p']} vs oracle on its inputs: " + " ".join(f"{k}={v:.2e}" for k, v in kern.items()))
+    say(f"    fp32 {c['n']}^3 p={c['p']} vs FP64 oracle: " + " ".join(f"{k}={v:.2e}" for k, v in e2e.items()))
+    for k, v in kern.items():
+        assert v <= 5e-5, (k, v)
+
+
+@pytest.mark.xfail(reason="FP32 storage of the velocity: the diffusive momentum residual carries "
+                          "4e-5 (128^3 p=4) and 2.5e-5 (48^3 p=7) against the FP64 oracle; "
+                          "the f32split form keeps u and pr in FP64 and meets the bar",
+                   strict=False)
+@pytest.mark.parametrize("form", ["mixed", "fp32"])
+def test_all_fp32_storage_forms_against_bar(gpu, precision_case, form):
+    c = precision_case
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, c["out"][form], c["want"])}
+    assert max(e2e.values()) <= BAR, e2e
+
+
+def sampled_oracle_recovered_p(q, nodes):
+    """sampled_oracle for the forms that recover the pressure from the rounded
+    conservative state, p = (gamma-1)(E - rho|u|^2/2), in FP64."""
+    g = q.gamma
+    q.pr = np.asfortranarray((g - 1.0) * (q.E - 0.5 * q.rho * np.sum(q.u * q.u, axis=1)))
+    return sampled_oracle(q, nodes)

@@ -145,15 +145,22 @@ def test_mixed_precision_error(gpu, p, n):
         assert e2e[k] <= 4 * TOL_MIXED, (k, e2e[k])
 
 
-TOL_FP32 = 1e-4  # stated bar of the FP32-arithmetic form at these sizes
+# Bar of the FP32 arithmetic itself (gathers, fluxes, sweeps in FP32), against
+# the FP64 oracle on the same FP32-rounded inputs; measured up to 1.3e-6 at
+# these sizes.  Against the oracle on the FP64 inputs these forms also carry
+# the FP32 rounding of the velocity, which exceeds the 1e-6 bar of the
+# precision split on large meshes (tests/test_gpu_full_configs.py, xfail);
+# the f32split form below meets it.
+TOL_FP32 = 5e-6
 
 
 @pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (7, 3)])
 def test_fp32_arithmetic_error(gpu, p, n):
     """FP32 storage and FP32 gathers, fluxes and derivative sweeps, FP64 node
-    sums and assembly: the error against the FP64 oracle (on the FP32-rounded
-    inputs, and on the FP64 inputs) is printed and stays within TOL_FP32; it
-    differs from the FP64-arithmetic form by more than rounding of the sums."""
+    sums and assembly: the error against the FP64 oracle on the FP32-rounded
+    inputs stays within TOL_FP32 (printed with the error on the FP64
+    inputs); it differs from the FP64-arithmetic form by more than rounding
+    of the sums."""
     say(f"[gpu] FP32 arithmetic p={p} n={n}")
     prob = m.box_problem(n, p, jitter=0.05)
     o = Oracle()
@@ -167,8 +174,50 @@ def test_fp32_arithmetic_error(gpu, p, n):
     say("    vs FP64 oracle on the FP32 inputs: " + " ".join(f"{k}={v:.2e}" for k, v in kern.items()))
     for k in KEYS:
         assert kern[k] <= TOL_FP32, (k, kern[k])
-        assert e2e[k] <= TOL_FP32, (k, e2e[k])
     assert max(kern.values()) > 1e-10  # the sweeps really ran in FP32
+
+
+TOL_SPLIT = 1e-6  # north-star bar of the FP32 split against the FP64 oracle
+
+
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (7, 3)])
+def test_f32split_error(gpu, p, n):
+    """FP32 He, gpvol, mu_e, mu_sgs, q, rho, E, Tem, mu_fluid; FP64 u and pr;
+    FP64 arithmetic: equal to the FP64 oracle on the same rounded inputs
+    (1e-12) and within TOL_SPLIT of the oracle on the FP64 inputs."""
+    import copy
+    say(f"[gpu] f32split p={p} n={n}")
+    prob = m.box_problem(n, p, jitter=0.05)
+    o = Oracle()
+    ref = o.convec(prob, 4) + o.diffu(prob, 4)
+    q = copy.copy(prob)
+    for k in ("He", "gpvol", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs"):
+        setattr(q, k, np.asfortranarray(getattr(prob, k).astype(np.float32).astype(np.float64)))
+    refs = o.convec(q, 4) + o.diffu(q, 4)
+    t = prob.tables
+    f = prob.fp32()
+    R, D = prob.outputs(), prob.outputs()
+    ins = (prob.connec, f["He"], f["gpvol"], prob.u, f["q"], f["rho"], prob.pr, f["E"], f["Tem"],
+           f["mu_fluid"], f["mu_e"], f["mu_sgs"])
+    m.copyin(*ins)
+    m.create(*R, *D)
+    try:
+        m.full_convec_ijk_f32split(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip,
+                                   t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.u, f["q"], f["rho"], prob.pr,
+                                   f["E"], *R)
+        m.full_diffusion_ijk_f32split(prob.nelem, prob.npoin, prob.connec, f["He"], f["gpvol"], t.dlxigp_ip,
+                                      t.invAtoIJK, t.AtoI, t.AtoJ, t.AtoK, prob.Cp, prob.Pr, f["rho"], prob.u,
+                                      f["Tem"], f["mu_fluid"], f["mu_e"], f["mu_sgs"], *D)
+        m.update_host(*R, *D)
+    finally:
+        m.delete(*ins, *R, *D)
+    out = R + D
+    e2e = {k: rel_l2(g, w) for k, g, w in zip(KEYS, out, ref)}
+    kern = {k: rel_l2(g, w) for k, g, w in zip(KEYS, out, refs)}
+    say("    vs FP64 oracle on FP64 inputs: " + " ".join(f"{k}={v:.2e}" for k, v in e2e.items()))
+    for k in KEYS:
+        assert kern[k] <= TOL64, (k, kern[k])
+        assert e2e[k] <= TOL_SPLIT, (k, e2e[k])
 
 
 def gpu_residuals_geo32(prob):
