@@ -2,8 +2,8 @@
 // src/kernels/sources/elem_convec.f90:198-431, with the energy fluxes of
 // full_convec_ijk_split, :674-832).
 //
-// One workgroup per batch of Shape<P>::BATCH elements, processed in passes of
-// Shape<P>::EPB elements with one thread per element node.  Per pass:
+// One workgroup per batch of CShape<P>::BATCH elements, processed in passes of
+// CShape<P>::EPB elements with one thread per element node.  Per pass:
 //  1. gather: each thread reads its global node (conn, element-major) and
 //     writes 10 nodal values to LDS:
 //       rho, p, H = E + k, rhoH = rho E + rho k, u(3), q(3)     (k = |u|^2/2)
@@ -64,10 +64,10 @@ __device__ __forceinline__ void convec_fetch(const A &a, int64_t e, int l, uint3
 // and sweeps (FP64, or FP32 for the FP32-arithmetic form); the node sums and
 // the assembly stay FP64.
 template <int P, class A, typename TA = double>
-__global__ void __launch_bounds__(Shape<P>::NT, 3) k_convec(A a)
+__global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
 {
     using TG = typename A::G;
-    using S = Shape<P>;
+    using S = CShape<P>;
     using T = TA;  // arithmetic
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX, NPASS = S::NPASS;
     constexpr int NF = 10;
@@ -258,7 +258,7 @@ template <int P, class A, typename TA = double>
 static hipError_t launch_p(const A &a, hipStream_t s)
 {
     if (a.plan.nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_convec<P, A, TA>), dim3(plan_blocks(a.plan)), dim3(Shape<P>::NT), 0, s, a);
+    hipLaunchKernelGGL((k_convec<P, A, TA>), dim3(plan_blocks(a.plan)), dim3(CShape<P>::NT), 0, s, a);
     return hipGetLastError();
 }
 

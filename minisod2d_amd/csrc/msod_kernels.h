@@ -56,8 +56,26 @@ struct Shape {
     static constexpr int UMAX = BATCH * NN;
 };
 
-// Host-side copy of Shape<P>::BATCH for run-time porder.
+// The convection's batches: Shape<P>'s, except from porder 6 on, where one
+// element fills a workgroup; there two elements per batch (two passes) share
+// a face in the node sums and halve the write phases per element (48^3 p=7:
+// 4.75 -> 4.01 ms).  The diffusion measured slower with two (3.58 -> 3.81 ms):
+// each kernel has its own plan where the sizes differ.
+template <int P>
+struct CShape {
+    static constexpr int Q = Shape<P>::Q;
+    static constexpr int NN = Shape<P>::NN;
+    static constexpr int EPB = Shape<P>::EPB;
+    static constexpr int NT = Shape<P>::NT;
+    static constexpr int BATCH = NN >= 343 ? 2 * EPB : Shape<P>::BATCH;
+    static constexpr int NPASS = BATCH / EPB;
+    static constexpr int UMAX = BATCH * NN;
+};
+
+// Host-side copies of Shape<P>::BATCH (diffusion and closure kernels) and
+// CShape<P>::BATCH (convection) for run-time porder.
 int batch_size(int porder);
+int convec_batch_size(int porder);
 
 struct Tables {
     const double *D;      // [q*q]

@@ -231,6 +231,21 @@ int batch_size(int porder)
     }
 }
 
+int convec_batch_size(int porder)
+{
+    switch (porder) {
+    case 1: return CShape<1>::BATCH;
+    case 2: return CShape<2>::BATCH;
+    case 3: return CShape<3>::BATCH;
+    case 4: return CShape<4>::BATCH;
+    case 5: return CShape<5>::BATCH;
+    case 6: return CShape<6>::BATCH;
+    case 7: return CShape<7>::BATCH;
+    case 8: return CShape<8>::BATCH;
+    default: return 1;
+    }
+}
+
 template <typename T>
 hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
                             const int32_t *l2g, T *geo, T *gvol, hipStream_t s)
@@ -566,7 +581,7 @@ int upload(const std::vector<V> &h, V **d, const char *what)
 
 // Builds the assembly plan of a mesh (msod_plan.h) from its connectivity,
 // a device array in the reference layout, and uploads it.
-int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet *ts, PlanEntry &P)
+int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet *ts, int B, PlanEntry &P)
 {
     const int64_t E = nelem * (int64_t)ts->l2g.size();
     std::vector<int32_t> raw(E);
@@ -574,7 +589,7 @@ int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet
            "reading connec");
     HostPlan H;
     std::string err;
-    if (!build_host_plan(raw.data(), nelem, npoin, ts->l2g, batch_size(ts->porder), H, &err))
+    if (!build_host_plan(raw.data(), nelem, npoin, ts->l2g, B, H, &err))
         return fail(MSOD_ERR_MESH, "%s", err.c_str());
     std::vector<int32_t>().swap(raw);
     P.B = H.B;
@@ -620,13 +635,13 @@ void drop_unversioned(const void *src)
         }
 }
 
-int get_plan(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, const TableSet *ts,
+int get_plan(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, const TableSet *ts, int B,
              const PlanEntry **out)
 {
     if (ver == 0) drop_unversioned(dconnec);
     for (auto &c : g.plans)
         if (c.src == dconnec && c.ver == ver && c.nelem == nelem && c.npoin == npoin &&
-            c.porder == ts->porder && c.l2g == ts->l2g) {
+            c.porder == ts->porder && c.l2g == ts->l2g && c.B == B) {
             *out = &c;
             return MSOD_OK;
         }
@@ -637,7 +652,7 @@ int get_plan(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, co
     P.npoin = npoin;
     P.porder = ts->porder;
     P.l2g = ts->l2g;
-    int st = build_plan(dconnec, nelem, npoin, ts, P);
+    int st = build_plan(dconnec, nelem, npoin, ts, B, P);
     if (st) {
         P.release();
         return st;
@@ -895,7 +910,7 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     if ((st = resolve(Rener, sizeof(TO) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
     const TG *geo, *gvol;
-    if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
+    if ((st = get_plan(dc, vc, NE, NP, ts, convec_batch_size(porder), &plan))) return st;
     if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
     ConvecArgs<TG, TF, TO, TU, TP> a;
@@ -956,7 +971,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     if ((st = resolve(Rener, sizeof(TO) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
     const TG *geo, *gvol;
-    if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
+    if ((st = get_plan(dc, vc, NE, NP, ts, batch_size(porder), &plan))) return st;
     if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
 
     DiffuArgs<TG, TF, TO, TU> a;
@@ -1100,7 +1115,7 @@ int closure_elem_common(int porder, int nelem, int npoin, const int32_t *connec,
     if ((st = resolve(u, sizeof(double) * 3 * NP, "u", &du, &vx))) return st;
     const PlanEntry *plan;
     const double *geo, *gvol;
-    if ((st = get_plan(dc, vc, NE, NP, ts, &plan))) return st;
+    if ((st = get_plan(dc, vc, NE, NP, ts, batch_size(porder), &plan))) return st;
     if ((st = get_geo<double>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
     a = ClosureElemArgs{};
     a.nelem = NE;

@@ -77,6 +77,15 @@ def run_pair(prob):
 
 def sampled_oracle(prob, nodes):
     """Oracle residuals at the 1-based `nodes`, from the elements touching them."""
+    sub, pos = sampled_problem(prob, nodes)
+    o = Oracle()
+    want = o.convec(sub, 8) + o.diffu(sub, 8)
+    return [w[pos] for w in want]
+
+
+def sampled_problem(prob, nodes):
+    """The elements touching the 1-based `nodes` as a problem of their own, and
+    the positions of `nodes` in its node numbering."""
     mark = np.zeros(prob.npoin + 1, dtype=bool)
     mark[nodes] = True
     hit = mark[prob.connec].any(axis=1)
@@ -90,10 +99,7 @@ def sampled_oracle(prob, nodes):
                     *[f(getattr(prob, k)[uniq - 1]) for k in
                       ("u", "q", "rho", "pr", "E", "Tem", "mu_fluid")],
                     f(prob.mu_e[els]), f(prob.mu_sgs[els]), Cp=prob.Cp, Pr=prob.Pr, gamma=prob.gamma)
-    o = Oracle()
-    want = o.convec(sub, 8) + o.diffu(sub, 8)
-    pos = np.searchsorted(uniq, nodes)
-    return [w[pos] for w in want]
+    return sub, np.searchsorted(uniq, nodes)
 
 
 def uniform_state(prob):
@@ -268,3 +274,74 @@ def sampled_oracle_recovered_p(q, nodes):
     g = q.gamma
     q.pr = np.asfortranarray((g - 1.0) * (q.E - 0.5 * q.rho * np.sum(q.u * q.u, axis=1)))
     return sampled_oracle(q, nodes)
+
+
+# ---------------------------------------------------------------------------
+# learned closure at the closure configuration's full size
+# ---------------------------------------------------------------------------
+def test_fused_closure_at_full_size(gpu):
+    """96^3 p=3: full_diffusion_ijk with mu_sgs predicted inside the kernel
+    (bf16 MFMA), at 2000 sampled nodes, against the oracle diffusion fed the
+    oracle MLP's mu_sgs on the features of the same elements (1e-4, the bound
+    of the bf16 network's rounding, as at small sizes); weights with a scale
+    that makes mu_sgs matter (up to half of mu_fluid)."""
+    from minisod2d_amd import closure as C
+    say("[gpu] closure 96^3 p=3: building the mesh")
+    prob = m.box_problem(96, 3, jitter=0.05, threads=THREADS)
+    t = prob.tables
+    nodes = np.random.default_rng(31).choice(prob.npoin, size=2000, replace=False) + 1
+    sub, pos = sampled_problem(prob, nodes)
+    # features of the sampled elements (they depend on the element's own nodes only)
+    F = np.zeros((sub.nelem * t.nnode, 12), np.float32)
+    ins = [sub.connec, sub.He, sub.gpvol, sub.rho, sub.u]
+    m.copyin(*ins)
+    m.create(F)
+    try:
+        m.closure_features(sub.nelem, sub.npoin, sub.connec, sub.He, sub.gpvol, t.dlxigp_ip, t.invAtoIJK,
+                           t.AtoI, t.AtoJ, t.AtoK, sub.rho, sub.u, F)
+        m.update_host(F)
+    finally:
+        m.delete(*ins, F)
+    rng = np.random.default_rng(5)
+    P = np.zeros(m.CLOSURE_NPARAM, np.float32)
+    P[C.OW1:C.OB1] = rng.normal(0, 0.5, 12 * 32)
+    P[C.OB1:C.OW2] = rng.normal(0, 0.2, 32)
+    P[C.OW2:C.OB2] = rng.normal(0, 0.3, 32 * 32)
+    P[C.OB2:C.OW3] = rng.normal(0, 0.2, 32)
+    P[C.OW3:C.OB3] = rng.normal(0, 0.5, 32)
+    P[C.OB3] = -0.3
+    f64 = F.astype(np.float64)
+    P[C.OMEAN:C.OISTD] = f64.mean(0)
+    sd = f64.std(0)
+    P[C.OISTD:C.OSCALE] = np.where(sd > 0, 1.0 / np.where(sd > 0, sd, 1.0), 1.0)
+    P[C.OSCALE] = 5e-4
+    say("[gpu] closure 96^3 p=3: fused kernel on the whole mesh")
+    R = prob.outputs()
+    ins = [prob.connec, prob.He, prob.gpvol, prob.rho, prob.u, prob.Tem, prob.mu_fluid, prob.mu_e]
+    m.copyin(*ins)
+    m.create(*R)
+    try:
+        used = m.full_diffusion_ijk_closure(prob.nelem, prob.npoin, prob.connec, t.Ngp, t.dNgp, prob.He,
+                                            prob.gpvol, t.dlxigp_ip, t.xgp, t.invAtoIJK, t.AtoI, t.AtoJ,
+                                            t.AtoK, prob.Cp, prob.Pr, prob.rho, prob.u, prob.Tem, prob.mu_fluid,
+                                            prob.mu_e, P, *R)
+        m.update_host(*R)
+    finally:
+        m.delete(*ins, *R)
+    assert used
+    say("[gpu] closure 96^3 p=3: oracle MLP and diffusion at the sampled nodes")
+    o = Oracle()
+    _, muo = o.mlp_forward(P, F, quant=1)
+    l2g = t.invAtoIJK.ravel(order="F") - 1
+    mu = np.zeros((sub.nelem, t.nnode), order="F")
+    mu[:, l2g] = muo.reshape(sub.nelem, t.nnode)
+    assert float(mu.max()) > 0.1 * float(prob.mu_fluid.max())  # the closure matters here
+    want = o.diffu(sub, 8, mu_sgs=mu)
+    zero = o.diffu(sub, 8, mu_sgs=np.zeros_like(mu))
+    for key, got, w, z in zip(("Dmass", "Dmom", "Dener"), R, want, zero):
+        err = rel_l2(got[nodes - 1], w[pos])
+        dz = rel_l2(z[pos], w[pos])
+        say(f"    {key}: fused vs oracle {err:.2e} (mu_sgs = 0 differs by {dz:.2e})")
+        assert err <= 1e-4, (key, err)
+        if key != "Dmass":  # the mass diffusion does not depend on mu_sgs (elem_diffu.f90:143,183)
+            assert dz > 100 * err, (key, dz, err)

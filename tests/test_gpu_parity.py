@@ -458,6 +458,10 @@ def test_fortran_driver(gpu, tmp_path, args, case):
     timers = np.loadtxt(tmp_path / "timers.dat")
     assert timers.shape == (3, 5) and np.all(timers[:, 1] > 0) and np.all(timers[:, 3] > 0)
     assert "GLL-node updates/s" in r.stdout
+    # the timing lines of main.f90:504-534, the split form's included
+    for name in ("Avg. convective time", "Avg. split convective time", "Max. split convective time",
+                 "Min. split convective time", "Variation split convec.", "Avg. diffusive time"):
+        assert re.search(rf"{re.escape(name)}\s*=\s*[-+0-9.Ee]+", r.stdout), name
     # the summary block of main.f90:544-566 quotes the extremes of the dumps
     summary = r.stdout[r.stdout.index("Basic results:"):].split("Basic TET results:")[0]
     cols = {"Rmass": (rc, 1), "Rmom(:,1)": (rc, 2), "Rmom(:,2)": (rc, 3), "Rmom(:,3)": (rc, 4),
