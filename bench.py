@@ -415,6 +415,100 @@ def sampled_parity(prob, out, nsample=3000, seed=7):
     return errs
 
 
+SPLIT_NOTES = {
+    "f32split": "FP32 He, gpvol, mu_e, mu_sgs, q, rho, E, Tem, mu_fluid; FP64 u and pr "
+                "(chosen field by field for <= 1e-6); FP64 arithmetic and atomic assembly",
+    "mixed": "every input stored in FP32, FP64 arithmetic and assembly; error = FP32 "
+             "rounding of the inputs, amplified by the derivative sweeps",
+    "geo32": "He, gpvol, mu_e, mu_sgs stored in FP32, nodal fields FP64, FP64 "
+             "arithmetic and assembly",
+    "fp32": "every input stored in FP32, FP32 gathers, fluxes and derivative sweeps; "
+            "FP64 node sums, atomic assembly and pressure recovery"}
+
+
+def split_record(E, N, r64, r, variant):
+    """One precision form against the FP64 run of the same mesh."""
+    wb = (4, 8, 4) if variant == "geo32" else (4, 4, 4)
+    bcm, bdm = split_bytes(E, N) if variant == "f32split" else canonical_bytes(E, N, *wb)
+    errs = r.get("sampled_rel_l2")
+    return {"value": N / (r["ms_per_step"] * 1e-3), "ms_per_step": r["ms_per_step"],
+            "speedup_vs_fp64": r64["ms_per_step"] / r["ms_per_step"],
+            "kernel_speedup_vs_fp64": (r64["convec_ms"] + r64["diffu_ms"]) / (r["convec_ms"] + r["diffu_ms"]),
+            "kernels_ms": {"convec": r["convec_ms"], "diffu": r["diffu_ms"]},
+            "hbm_frac_convec": bcm / (r["convec_ms"] * 1e-3) / HBM_PEAK,
+            "hbm_frac_diffu": bdm / (r["diffu_ms"] * 1e-3) / HBM_PEAK,
+            "rel_l2_vs_fp64_oracle": errs,
+            "max_rel_l2": max(errs.values()) if errs else None,
+            "storage": SPLIT_NOTES[variant]}
+
+
+def config_record(n, p, jitter, jump, steps, warmup, threads, variants=("f32split", "mixed", "fp32"),
+                  parity=True):
+    """A second mesh of the benchmark set, timed in FP64 and in the precision
+    forms, with each form's sampled error against the FP64 oracle."""
+    log(f"config {n}^3 porder {p} jitter {jitter} density jump {jump}: generating")
+    prob = m.box_problem(n, p, jitter=jitter, rho_jump=jump, threads=threads)
+    N, E = prob.npoin, prob.nelem * prob.tables.nnode
+    R, D = prob.outputs(), prob.outputs()
+    ins = prob.device_inputs()
+    m.copyin(*ins)
+    m.create(*R, *D)
+    f32 = prob.fp32()
+    f32_ins = [f32[k] for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")]
+    try:
+        r64 = timed(prob, R, D, steps, warmup, "fp64")
+        log(f"config {n}^3 p={p} FP64 {r64}")
+        errs = None
+        if parity:
+            m.update_host(*R, *D)
+            errs = sampled_parity(prob, R + D, nsample=2000)
+        bc, bd = canonical_bytes(E, N, 8, 8)
+        rec = {"workload": f"unit cube {n}x{n}x{n} hex elements, porder {p}, {N} GLL nodes, "
+                           f"{E} element nodes, jitter {jitter}, density jump {jump}",
+               "npoin": N, "nelem": prob.nelem, "porder": p,
+               "fp64": {"value": N / (r64["ms_per_step"] * 1e-3), "ms_per_step": r64["ms_per_step"],
+                        "kernels_ms": {"convec": r64["convec_ms"], "diffu": r64["diffu_ms"],
+                                       "zero_fill": r64["zero_ms"]},
+                        "hbm_frac_convec": bc / (r64["convec_ms"] * 1e-3) / HBM_PEAK,
+                        "hbm_frac_diffu": bd / (r64["diffu_ms"] * 1e-3) / HBM_PEAK,
+                        "sampled_rel_l2_vs_oracle": errs}}
+        m.copyin(*f32_ins)
+        for variant in variants:
+            r = timed(prob, R, D, steps, warmup, variant, f32=f32)
+            if parity:
+                m.update_host(*R, *D)
+                r["sampled_rel_l2"] = sampled_parity(prob, R + D, nsample=2000)
+            rec[variant] = split_record(E, N, r64, r, variant)
+            log(f"config {n}^3 p={p} {variant} {rec[variant]['ms_per_step']:.3f} ms, "
+                f"max err {rec[variant]['max_rel_l2']}")
+    finally:
+        m.delete(*ins, *R, *D, *f32_ins)
+    return rec
+
+
+def closure_record(n=96, seed=50, repeats=10):
+    """The learned closure at porder 3 on an n^3 box: the fused inference +
+    diffusion kernel against the plain diffusion (mu_sgs read from memory), and
+    the error of the order-3 residual against the order-7 residual of the same
+    fields projected to order 3, with the closure and with mu_sgs = 0."""
+    from minisod2d_amd import closure as cl
+    try:
+        w = cl.load_weights()
+    except cl.ClosureModelError as e:
+        return {"error": str(e)}
+    log(f"closure: order-7/order-3 pair on {n}^3 elements")
+    t0 = time.time()
+    pair = cl.make_pair(n, seed=seed)
+    rec = cl.report(pair, w["params"], repeats=repeats)
+    rec["weights_sha256"] = w["manifest"].get("sha256")
+    rec["pair_build_s"] = round(time.time() - t0, 1)
+    rec["note"] = ("closure_ms: mu_sgs inferred per element node by the MLP (bf16 MFMA) inside the "
+                   "diffusion kernel; plain_ms: full_diffusion_ijk with mu_sgs from memory; errors per "
+                   "component (momentum x, y, z, energy) relative to the projected order-7 residual")
+    del pair
+    return rec
+
+
 def cpu_model():
     """The host CPU's model name (/proc/cpuinfo), for the baseline record."""
     try:
@@ -481,6 +575,8 @@ def main():
                     help="also time the other assembly mode (FP64)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 2 GiB device-to-device copy first (HBM counter calibration)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 48^3 porder 7 precision record and the 96^3 porder 3 closure record")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print HBM bytes per dispatch from two counter runs and exit")
     args = ap.parse_args()
@@ -589,27 +685,20 @@ def main():
                                  "ms_per_step": other["ms_per_step"],
                                  "kernels_ms": {"convec": other["convec_ms"], "diffu": other["diffu_ms"],
                                                 "zero_fill": other["zero_ms"]}}
-    notes = {"f32split": "FP32 He, gpvol, mu_e, mu_sgs, q, rho, E, Tem, mu_fluid; FP64 u and pr "
-                         "(chosen field by field for <= 1e-6); FP64 arithmetic and atomic assembly",
-             "mixed": "every input stored in FP32, FP64 arithmetic and assembly; error = FP32 "
-                      "rounding of the inputs, amplified by the derivative sweeps",
-             "geo32": "He, gpvol, mu_e, mu_sgs stored in FP32, nodal fields FP64, FP64 "
-                      "arithmetic and assembly",
-             "fp32": "every input stored in FP32, FP32 gathers, fluxes and derivative sweeps; "
-                     "FP64 node sums, atomic assembly and pressure recovery"}
     for variant, r in split.items():
-        wb = (4, 8, 4) if variant == "geo32" else (4, 4, 4)
-        bcm, bdm = split_bytes(E, N) if variant == "f32split" else canonical_bytes(E, N, *wb)
-        rec[variant] = {"value": N / (r["ms_per_step"] * 1e-3), "ms_per_step": r["ms_per_step"],
-                        "speedup_vs_fp64": r64["ms_per_step"] / r["ms_per_step"],
-                        "kernels_ms": {"convec": r["convec_ms"], "diffu": r["diffu_ms"]},
-                        "hbm_frac_convec": bcm / (r["convec_ms"] * 1e-3) / HBM_PEAK,
-                        "hbm_frac_diffu": bdm / (r["diffu_ms"] * 1e-3) / HBM_PEAK,
-                        "rel_l2_vs_fp64_oracle": r.get("sampled_rel_l2"),
-                        "max_rel_l2": (max(r["sampled_rel_l2"].values()) if r.get("sampled_rel_l2") else None),
-                        "storage": notes[variant]}
-    m.finalize()
+        rec[variant] = split_record(E, N, r64, r, variant)
+    m.delete(*ins, *R, *D)
+    if split:
+        m.delete(*[f32[k] for k in ("He", "gpvol", "u", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")])
+        del f32
     del prob, R, D, ins
+    if not args.no_configs:
+        threads = min(args.threads, os.cpu_count() or 1)
+        rec["cfg3"] = config_record(48, 7, 0.05, 0.1, args.steps, args.warmup, threads,
+                                    parity=not args.no_parity)
+        rec["closure"] = closure_record()
+        log(f"closure {rec['closure']}")
+    m.finalize()
     if not args.no_cpu_baseline:
         log("timing the reference CPU kernels on a bounded sample")
         rec["cpu_baseline"] = cpu_baseline(p)

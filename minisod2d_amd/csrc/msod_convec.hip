@@ -170,11 +170,13 @@ __global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
             // once: non-temporal loads keep them from evicting the gathered
             // nodal lines, which neighbouring elements read again, from L2.
             const int64_t ec = elem(b, p);
+            // kept in the storage type until used: a conversion here would
+            // wait for the load before the barrier
             const TG *geo = a.geo + ec * 9 * NN + l;
-            T hh[9];
+            TG hh[9];
 #pragma unroll
             for (int c = 0; c < 9; ++c) hh[c] = __builtin_nontemporal_load(&geo[c * NN]);
-            const T gv = __builtin_nontemporal_load(&a.gvol[ec * NN + l]);
+            const TG gvr = __builtin_nontemporal_load(&a.gvol[ec * NN + l]);
             convec_fetch<NN>(a, elem_of(g + 1), l, gnext, nxt);
             gnext = __builtin_nontemporal_load(&pl.conn[elem_of(g + 2) * NN + l]);
             __syncthreads();
@@ -192,9 +194,9 @@ __global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
                     // one copy of the sweep code for the three directions
-                    const T h0 = d == 0 ? hh[0] : (d == 1 ? hh[3] : hh[6]);
-                    const T h1 = d == 0 ? hh[1] : (d == 1 ? hh[4] : hh[7]);
-                    const T h2 = d == 0 ? hh[2] : (d == 1 ? hh[5] : hh[8]);
+                    const T h0 = (T)(d == 0 ? hh[0] : (d == 1 ? hh[3] : hh[6]));
+                    const T h1 = (T)(d == 0 ? hh[1] : (d == 1 ? hh[4] : hh[7]));
+                    const T h2 = (T)(d == 0 ? hh[2] : (d == 1 ? hh[5] : hh[8]));
                     const int Id = d == 0 ? I : (d == 1 ? J : K);
                     const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
                     const int base = l - Id * stride;
@@ -240,6 +242,7 @@ __global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     Rmo[i] = (T)0.25 * (aM[i] + uu[i] * aQ + qq[i] * aU + uu[i] * sRho + mM[i]);
+                const T gv = (T)gvr;
                 v[0] = (double)(gv * Rm);
                 v[1] = (double)(gv * Rmo[0]);
                 v[2] = (double)(gv * Rmo[1]);

@@ -216,18 +216,29 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
             const int64_t en = e + EPB;
             if (le < EPB && en < a.nelem) diffu_fetch<NN, CLOSURE>(a, en, l, nxt);
         }
-        __syncthreads();
-
-        // ---- pass 1: gradients, stress ------------------------------------
-        T h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // h[i][d] = He(i,d)
-        T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        T tauU[3] = {0, 0, 0}, gv = 0, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        if (active) {
-            const TG *geo = a.geo + e * 9 * NN + l;  // read once: non-temporal, spares L2
+        // this pass's element arrays, issued before the barrier (read once:
+        // non-temporal, spares L2); unconditional, idle threads read a real
+        // element; kept in the storage type until used
+        TG hr[3][3], gvr;
+        uint32_t li;
+        {
+            const int64_t ec = e < a.nelem ? (le < EPB ? e : b * S::BATCH + pass * EPB) : a.nelem - 1;
+            const TG *geo = a.geo + ec * 9 * NN + l;
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) h[i][d] = __builtin_nontemporal_load(&geo[(i + 3 * d) * NN]);
+                for (int i = 0; i < 3; ++i) hr[i][d] = __builtin_nontemporal_load(&geo[(i + 3 * d) * NN]);
+            gvr = __builtin_nontemporal_load(&a.gvol[ec * NN + l]);
+            li = __builtin_nontemporal_load(&pl.lidx[ec * NN + l]);
+        }
+        __syncthreads();
+
+        // ---- pass 1: gradients, stress ------------------------------------
+        // He(i,d) = hr[i][d], converted where used: a conversion placed
+        // ahead of the sweeps would wait for every load there
+        T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        T tauU[3] = {0, 0, 0}, gv = 0, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (active) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int Id = d == 0 ? I : (d == 1 ? J : K);
@@ -247,12 +258,13 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    gR[i] += h[i][d] * aR;
-                    gT[i] += h[i][d] * aT;
+                    const T hid = (T)hr[i][d];
+                    gR[i] += hid * aR;
+                    gT[i] += hid * aT;
                     // G[i][j] = d u_i / d x_j = sum_d He(j,d) D_d[u_i]
-                    G[0][i] += h[i][d] * aU0;
-                    G[1][i] += h[i][d] * aU1;
-                    G[2][i] += h[i][d] * aU2;
+                    G[0][i] += hid * aU0;
+                    G[1][i] += hid * aU1;
+                    G[2][i] += hid * aU2;
                 }
             }
         }
@@ -260,7 +272,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
             __syncthreads();  // every pass-1 sweep over sN done: the MLP scratch overlays it
             float f[kMlpIn];
             if (active)
-                mlp_features<T>(G, r, uu, h, f);
+                mlp_features<T>(G, r, uu, hr, f);
             else
 #pragma unroll
                 for (int k = 0; k < kMlpIn; ++k) f[k] = smlp.mean[k];
@@ -283,7 +295,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
                 tauU[i] = acc - lam * uu[i];
                 tau[i][i] -= lam;
             }
-            gv = __builtin_nontemporal_load(&a.gvol[e * NN + l]);
+            gv = (T)gvr;
         }
 
         // ---- pass 2: transposed sweeps, one reference direction at a time -
@@ -292,7 +304,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (active) {
-                const T c0 = gv * h[0][d], c1 = gv * h[1][d], c2 = gv * h[2][d];
+                const T c0 = gv * (T)hr[0][d], c1 = gv * (T)hr[1][d], c2 = gv * (T)hr[2][d];
                 sN[le][l][0] = V2{c0 * gR[0] + c1 * gR[1] + c2 * gR[2], c0 * tauU[0] + c1 * tauU[1] + c2 * tauU[2]};
                 sN[le][l][1] = V2{c0 * gT[0] + c1 * gT[1] + c2 * gT[2], c0 * tau[0][0] + c1 * tau[0][1] + c2 * tau[0][2]};
                 sN[le][l][2] = V2{c0 * tau[1][0] + c1 * tau[1][1] + c2 * tau[1][2], c0 * tau[2][0] + c1 * tau[2][1] + c2 * tau[2][2]};
@@ -326,7 +338,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
             v[3] = (double)W[5];
             v[4] = (double)(W[1] + kappa * W[2]);
         }
-        add_pass<UMAX, EPB>(sR, pl.mode, le, active, active ? (uint32_t)__builtin_nontemporal_load(&pl.lidx[e * NN + l]) : 0u, v);
+        add_pass<UMAX, EPB>(sR, pl.mode, le, active, li, v);
         if (pass + 1 < S::NPASS) cur = nxt;
     }
     __syncthreads();  // every node sum of the batch done

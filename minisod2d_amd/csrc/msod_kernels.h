@@ -38,8 +38,9 @@
 namespace msod {
 
 constexpr int kMaxP = 8;
-constexpr int kMaxKpb = 64;           // batches per workgroup, at most
-constexpr int kBatchesPerBlock = 8;   // batches per convection workgroup (measured best of 1..16 at p=4)
+constexpr int kMaxKpb = 16;           // batches per workgroup, at most (LDS list of the block's batches)
+constexpr int kBatchesPerBlock = 8;   // batches per workgroup (convection: measured best of 1..16 at p=4)
+static_assert(kBatchesPerBlock <= kMaxKpb, "the block's batch list holds kpb entries");
 enum AssemblyMode { kColored = 0, kAtomic = 1 };
 
 template <int P>

@@ -826,12 +826,19 @@ int check_sizes(int porder, int nelem, int npoin)
 // ms; colour order within chunks of 256..16384 batches: 13.07..19.90 ms).
 constexpr bool kConvecNatural = false;
 constexpr bool kDiffuNatural = true;
+// Batches per workgroup, at most, per kernel.  The diffusion takes one: a
+// form that ran kpb batches per workgroup with every load one pass ahead
+// needed 148-152 VGPRs against 120 (3 waves per SIMD against 4) and ran
+// slower (128^3 p=4: 14.4-14.9 against 13.4 ms for kpb 2..16; 48^3 p=7: 5.2
+// against 3.7 ms).
+constexpr int kConvecKpb = kBatchesPerBlock;
+constexpr int kDiffuKpb = 1;
 
 // Zero-fill (atomic mode) or unreferenced-node zeroing (coloured mode), then
 // the kernel: one launch per colour, or one launch over all batches, taken
 // in mesh order (natural) or grouped by colour.
 template <class A, class L>
-int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *what, bool natural)
+int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *what, bool natural, int kpb_max)
 {
     Plan pl;
     pl.conn = P.conn;
@@ -839,11 +846,11 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     pl.uoff = P.uoff;
     pl.unode = P.unode;
     pl.mode = g.assembly;
-    // batches per convection workgroup: kBatchesPerBlock on large meshes,
-    // fewer where that would leave the launch short of 2048 workgroups (8 per
-    // CU); 8^3 p=3 has 64 batches, which kBatchesPerBlock put on 8 CUs
-    auto kpb_for = [](int64_t nb) {
-        return (int)std::max<int64_t>(1, std::min<int64_t>(kBatchesPerBlock, nb / 2048));
+    // batches per workgroup: kpb_max on large meshes, fewer where that would
+    // leave the launch short of 2048 workgroups (8 per CU); 8^3 p=3 has 64
+    // batches, which 8 per workgroup put on 8 CUs
+    auto kpb_for = [kpb_max](int64_t nb) {
+        return (int)std::max<int64_t>(1, std::min<int64_t>(kpb_max, nb / 2048));
     };
     const int64_t NP = a.npoin;
     if (g.assembly == kColored && !P.colored)
@@ -935,7 +942,7 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
                                     if (fp32_arith) return launch_convec_fp32(porder, x, g.stream);
                                 return launch_convec(porder, x, g.stream);
                             },
-                            "convection kernel launch", kConvecNatural)))
+                            "convection kernel launch", kConvecNatural, kConvecKpb)))
         return st;
     return finish_call("convection kernel");
 }
@@ -1002,7 +1009,7 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
                                     if (fp32_arith) return launch_diffu_fp32(porder, x, g.stream);
                                 return launch_diffu(porder, x, g.stream);
                             },
-                            "diffusion kernel launch", kDiffuNatural)))
+                            "diffusion kernel launch", kDiffuNatural, kDiffuKpb)))
         return st;
     return finish_call("diffusion kernel");
 }
