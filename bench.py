@@ -196,17 +196,23 @@ def run_pair(prob, R, D, variant, f32=None):
 
 
 def timed(prob, R, D, steps, warmup, variant, f32=None):
+    """Wall time per pair over `steps` pairs with the per-launch timers off
+    (their events are measurement, not work), then the kernels' HIP-event
+    times over as many pairs again."""
     m.set_async(True)
     for _ in range(warmup):
         run_pair(prob, R, D, variant, f32)
     m.synchronize()
-    m.timing_reset()
-    m.timing_enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         run_pair(prob, R, D, variant, f32)
     m.synchronize()
     t1 = time.perf_counter()
+    m.timing_reset()
+    m.timing_enable(True)
+    for _ in range(steps):
+        run_pair(prob, R, D, variant, f32)
+    m.synchronize()
     m.timing_enable(False)
     m.set_async(False)
     kc, kd = TIMERS[variant]
@@ -506,6 +512,14 @@ def closure_record(n=96, seed=50, repeats=10):
                    "diffusion kernel; plain_ms: full_diffusion_ijk with mu_sgs from memory; errors per "
                    "component (momentum x, y, z, energy) relative to the projected order-7 residual")
     del pair
+    # noise-free pairs (Taylor-Green velocity of wavenumber 2, smooth density
+    # and pressure): the least-squares constant mu_sgs and its gain, and a
+    # network trained on 12^3 and 16^3 pairs, evaluated on a held-out 20^3 pair
+    log("closure: noise-free pairs, constant fit and a trained network")
+    t0 = time.time()
+    d = cl.diagnose(2, [12, 16], [10], [20], steps=300)
+    rec["noise_free_k2"] = {"constant_fit": d["constant"], "model_gain": d["model"],
+                            "best_step": d["best_step"], "seconds": round(time.time() - t0, 1)}
     return rec
 
 
@@ -563,6 +577,7 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="elements per axis")
     ap.add_argument("--porder", type=int, default=4)
     ap.add_argument("--jitter", type=float, default=0.0)
+    ap.add_argument("--rho-jump", type=float, default=0.0, help="density jump at x > 0.5 (relative)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -591,7 +606,8 @@ def main():
     n, p = args.n, args.porder
     log(f"generating {n}^3 porder {p} (host, {args.threads} threads)")
     t0 = time.time()
-    prob = m.box_problem(n, p, jitter=args.jitter, threads=min(args.threads, os.cpu_count() or 1))
+    prob = m.box_problem(n, p, jitter=args.jitter, rho_jump=args.rho_jump,
+                         threads=min(args.threads, os.cpu_count() or 1))
     host_gen_s = time.time() - t0
     N, E = prob.npoin, prob.nelem * prob.tables.nnode
     log(f"generated npoin={N} elem-nodes={E} in {host_gen_s:.1f}s")
@@ -665,7 +681,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"unit cube {n}x{n}x{n} hex elements, porder {p}, "
-                               f"{N} GLL nodes, {E} element nodes, jitter {args.jitter}",
+                               f"{N} GLL nodes, {E} element nodes, jitter {args.jitter}"
+                               + (f", density jump {args.rho_jump}" if args.rho_jump else ""),
                    "npoin": N, "nelem": prob.nelem, "porder": p},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,

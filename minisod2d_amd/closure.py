@@ -40,8 +40,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (CLOSURE_NFEAT, CLOSURE_NPARAM, CLOSURE_NTRAIN, SEED_JITTER, SEED_MUE, SEED_MUSGS,
-               SEED_RHO, Problem, box_problem, closure_adam, closure_features, closure_gradient,
+from . import (CLOSURE_NFEAT, CLOSURE_NPARAM, CLOSURE_NTRAIN, GAMMA, RGAS, SEED_JITTER, SEED_MUE,
+               SEED_MUSGS, SEED_RHO, Problem, box_problem, closure_adam, closure_features, closure_gradient,
                closure_loss, closure_predict, closure_sensitivity, copyin, create, delete,
                full_convec_ijk, full_diffusion_ijk, full_diffusion_ijk_closure, host_lib,
                update_device, update_host)
@@ -153,17 +153,47 @@ class ClosurePair:
         return self.coarse.nelem * self.coarse.tables.nnode
 
 
+def smooth_fields(prob: Problem, k: int = 2, mu_e: float = 5e-4) -> Problem:
+    """Replaces the synthetic fields by ones without per-node noise: the
+    Taylor-Green velocity of wavenumber k,
+        u = (sin a x cos a y cos a z, -cos a x sin a y cos a z, 0),  a = 2 pi k,
+    its pressure p = 1/gamma + (cos 2ax + cos 2ay)(cos 2az + 2)/16, a smooth
+    density rho = 1 + 0.05 sin a x sin a y sin a z, q = rho u,
+    E = p/(gamma-1) + rho|u|^2/2, T = p/(rho R), and a constant mu_e.  With the
+    noise gone, what the order-pt residual misses of the order-pf one is the
+    truncation error of under-resolved smooth flow, which an eddy viscosity
+    may or may not represent."""
+    x, y, z = (prob.coord[:, i] for i in range(3))
+    a = 2.0 * np.pi * k
+    u0 = np.sin(a * x) * np.cos(a * y) * np.cos(a * z)
+    u1 = -np.cos(a * x) * np.sin(a * y) * np.cos(a * z)
+    prob.u[:, 0] = u0
+    prob.u[:, 1] = u1
+    prob.u[:, 2] = 0.0
+    prob.rho[:] = 1.0 + 0.05 * np.sin(a * x) * np.sin(a * y) * np.sin(a * z)
+    prob.pr[:] = 1.0 / GAMMA + (np.cos(2 * a * x) + np.cos(2 * a * y)) * (np.cos(2 * a * z) + 2.0) / 16.0
+    prob.q[:] = prob.rho[:, None] * prob.u
+    prob.E[:] = prob.pr / (GAMMA - 1.0) + 0.5 * prob.rho * (u0 * u0 + u1 * u1)
+    prob.Tem[:] = prob.pr / (prob.rho * RGAS)
+    prob.mu_e[:] = mu_e
+    return prob
+
+
 def pair_seeds(seed: int):
     return (SEED_RHO + 17 * seed, SEED_MUE + 17 * seed, SEED_MUSGS + 17 * seed, SEED_JITTER)
 
 
 def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float = 0.0,
-              log=None) -> ClosurePair:
-    """Training pair on n^3 affine hexahedra (see the module docstring)."""
+              log=None, smooth_k: int = 0) -> ClosurePair:
+    """Training pair on n^3 affine hexahedra (see the module docstring);
+    smooth_k > 0: the noise-free fields of smooth_fields at wavenumber
+    smooth_k instead of the synthetic ones."""
     t0 = time.time()
     say = log or (lambda *_: None)
     seeds = pair_seeds(seed)
     fine = box_problem(n, pf, jitter=0.0, rho_jump=rho_jump, seeds=seeds)
+    if smooth_k:
+        smooth_fields(fine, smooth_k)
     say(f"  n={n}: order-{pf} problem built ({time.time() - t0:.1f} s)")
     fine.mu_sgs[:] = 0.0
     Rcf, Rdf = gpu_residuals(fine)
@@ -189,7 +219,7 @@ def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float =
     r0 = _mom_ener(Rd0[1], Rd0[2])
     gap = ((r0 - target) ** 2).sum(axis=1)
     weights = 1.0 / np.maximum(gap, np.finfo(float).tiny)
-    meta = dict(n=n, seed=seed, seeds=list(seeds), pf=pf, pt=pt, rho_jump=rho_jump,
+    meta = dict(n=n, seed=seed, seeds=list(seeds), pf=pf, pt=pt, rho_jump=rho_jump, smooth_k=smooth_k,
                 nelem=coarse.nelem, npoin=coarse.npoin, build_s=round(time.time() - t0, 2),
                 fine_norm=np.linalg.norm(_mom_ener(rt[:, 0:3], rt[:, 3]), axis=1).tolist())
     if log:
@@ -347,6 +377,14 @@ class _Device:
     def constant_optimum(self) -> float:
         """Least-squares constant mu_sgs of the pair (the loss is quadratic in
         a constant mu_sgs since the residual is affine in it)."""
+        return self.constant_fit()[0]
+
+    def constant_fit(self) -> tuple[float, float]:
+        """(c*, gain at c*): with L1 = Rd(mu_sgs = 1) - Rd(0), the loss of a
+        constant c is 4 - 2c num + c^2 den, num = sum w (target - r0) L1,
+        den = sum w L1^2; its minimum over c is at num/den with gain
+        num^2 / (4 den).  A closure, whose mu_sgs is >= 0, can only use a
+        positive c*."""
         p = self.pair.coarse
         one = np.ones((p.nelem, self.nn), order="F")
         copyin(one)
@@ -359,7 +397,9 @@ class _Device:
         w = self.pair.weights[:, None]
         num = float((w * (self.pair.target - self.pair.r0) * L1).sum())
         den = float((w * L1 * L1).sum())
-        return num / den if den > 0 else 0.0
+        if den <= 0:
+            return 0.0, 0.0
+        return num / den, num * num / (4.0 * den)
 
 
 def train(pairs: list[ClosurePair], steps: int = 600, lr: float = 3e-3, seed: int = 0,
@@ -466,6 +506,33 @@ def evaluate(pair: ClosurePair, params: np.ndarray | None) -> dict:
                 gain=float(1.0 - per.sum() / 4.0))
 
 
+def constant_fit(pair: ClosurePair) -> tuple[float, float]:
+    """Least-squares constant mu_sgs of a pair and its gain (_Device.constant_fit)."""
+    d = _Device(pair)
+    try:
+        return d.constant_fit()
+    finally:
+        d.close()
+
+
+def diagnose(k: int, sizes, val_sizes, held_out, steps: int = 300, lr: float = 3e-3, seed: int = 0,
+             log=None) -> dict:
+    """Noise-free pairs (smooth_fields at wavenumber k): the least-squares
+    constant mu_sgs and its gain on every pair, then a network trained on the
+    pairs of `sizes` (early stopping on `val_sizes`) and its gain on the
+    held-out sizes, which it never saw."""
+    mk = lambda ns, off: [make_pair(n, seed=seed + off + i, smooth_k=k, log=log) for i, n in enumerate(ns)]
+    groups = {"train": mk(sizes, 0), "val": mk(val_sizes, 100), "held_out": mk(held_out, 200)}
+    const = {g: [dict(n=p.meta["n"], c=c, gain=gn) for p in ps for c, gn in [constant_fit(p)]]
+             for g, ps in groups.items()}
+    res = train(groups["train"], steps=steps, lr=lr, seed=seed, log=log, val_pairs=groups["val"] or None)
+    gain = {g: [dict(n=p.meta["n"], gain=evaluate(p, res["params"])["gain"]) for p in ps]
+            for g, ps in groups.items()}
+    return dict(k=k, constant=const, model=gain, best_step=res.get("best_step"),
+                loss_first=res["history"][0], loss_last=res["history"][-1], params=res["params"],
+                adam_m=res["adam_m"], adam_v=res["adam_v"], step=res["step"])
+
+
 def report(pair: ClosurePair, params: np.ndarray, repeats: int = 10) -> dict:
     """Error against the projected fine residual with the closure and with
     mu_sgs = 0, and the cost of the fused closure kernel against the plain
@@ -535,6 +602,15 @@ def _main(argv=None) -> int:
     rp.add_argument("--n", type=int, default=96)
     rp.add_argument("--seed", type=int, default=50)
     rp.add_argument("--repeats", type=int, default=10)
+    dg = sub.add_parser("diagnose", help="noise-free pairs: constant optimum and trained gain")
+    dg.add_argument("--k", type=int, nargs="+", default=[1, 2, 3])
+    dg.add_argument("--sizes", type=int, nargs="+", default=[12, 16])
+    dg.add_argument("--val-sizes", type=int, nargs="*", default=[10])
+    dg.add_argument("--held-out", type=int, nargs="+", default=[20])
+    dg.add_argument("--steps", type=int, default=300)
+    dg.add_argument("--lr", type=float, default=3e-3)
+    dg.add_argument("--seed", type=int, default=0)
+    dg.add_argument("--out", default=None, help="weight file for the model of the last k")
     ev = sub.add_parser("evaluate")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ev.add_argument("--sizes", type=int, nargs="+", default=[28])
@@ -558,6 +634,16 @@ def _main(argv=None) -> int:
                     constant_optimum=res.get("constant_optimum"))
         sha = save_weights(a.out, res["params"], res["adam_m"], res["adam_v"], res["step"], meta)
         print(json.dumps(dict(out=a.out, sha256=sha, **meta)), flush=True)
+    elif a.cmd == "diagnose":
+        for k in a.k:
+            r = diagnose(k, a.sizes, a.val_sizes, a.held_out, steps=a.steps, lr=a.lr, seed=a.seed, log=log)
+            if a.out and k == a.k[-1]:
+                meta = dict(smooth_k=k, sizes=a.sizes, val_sizes=a.val_sizes, held_out=a.held_out,
+                            steps=r["step"], lr=a.lr, constant=r["constant"], model=r["model"])
+                r["sha256"] = save_weights(a.out, r["params"], r["adam_m"], r["adam_v"], r["step"], meta)
+            for key in ("params", "adam_m", "adam_v"):
+                r.pop(key)
+            print(json.dumps(r), flush=True)
     elif a.cmd == "report":
         w = load_weights(a.weights)
         pair = make_pair(a.n, seed=a.seed, log=log)

@@ -48,7 +48,9 @@ __device__ __forceinline__ float bf2f(short b)
 {
     return __uint_as_float(((uint32_t)(uint16_t)b) << 16);
 }
-__device__ __forceinline__ float sigmoidf_(float z) { return 1.f / (1.f + __expf(-z)); }
+// The hardware reciprocal (1 ulp) instead of a correctly rounded division:
+// the network runs in bf16, whose rounding is 2^-9.
+__device__ __forceinline__ float sigmoidf_(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 __device__ __forceinline__ float siluf_(float z) { return z * sigmoidf_(z); }
 __device__ __forceinline__ float dsiluf_(float z)
 {
@@ -113,13 +115,21 @@ __device__ __forceinline__ void mlp_features(const T (&Gij)[3][3], T rho, const 
     const double det = (double)hinv[0][0] * ((double)hinv[1][1] * hinv[2][2] - (double)hinv[1][2] * hinv[2][1]) -
                        (double)hinv[0][1] * ((double)hinv[1][0] * hinv[2][2] - (double)hinv[1][2] * hinv[2][0]) +
                        (double)hinv[0][2] * ((double)hinv[1][0] * hinv[2][1] - (double)hinv[1][1] * hinv[2][0]);
-    f[11] = (float)(2.0 * cbrt(1.0 / fabs(det)));  // det J = 1/det He
+    f[11] = 2.f * __builtin_amdgcn_rcpf(cbrtf(fabsf((float)det)));  // det J = 1/det He
+}
+
+// Orders this wave's LDS accesses (a wave's LDS operations complete in
+// issue order; the fence keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // z3 of the MLP for this lane's node (mu_sgs = scale*softplus(z3)).
-// scratch: this wave's LDS area of 64*32 shorts (4 KiB).  Contains
-// __syncthreads(): every thread of the block must call it, in uniform
-// control flow (inactive lanes pass any finite features).
+// scratch: this wave's LDS area of 64*32 shorts (4 KiB), used by this wave
+// only (no block barrier inside).  All 64 lanes of the wave call it
+// (inactive lanes pass any finite features).
 __device__ inline float mlp_forward_z3(const MlpShared &s, const float feat[kMlpIn], short *scratch)
 {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
@@ -127,7 +137,7 @@ __device__ inline float mlp_forward_z3(const MlpShared &s, const float feat[kMlp
 #pragma unroll
     for (int k = 0; k < 16; ++k)
         X[lane * 16 + k] = k < kMlpIn ? f2bf((feat[k] - s.mean[k]) * s.istd[k]) : (short)0;
-    __syncthreads();
+    wave_lds_sync();
     f32x4 z1[4][2];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
@@ -142,7 +152,7 @@ __device__ inline float mlp_forward_z3(const MlpShared &s, const float feat[kMlp
             z1[mt][nt] = mfma16(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
         }
     }
-    __syncthreads();
+    wave_lds_sync();
     short *H = scratch;  // [64][32], over X
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
@@ -153,7 +163,7 @@ __device__ inline float mlp_forward_z3(const MlpShared &s, const float feat[kMlp
                 const int n = nt * 16 + r;
                 H[(mt * 16 + 4 * g + i) * kMlpH + n] = f2bf(siluf_(z1[mt][nt][i] + s.b1[n]));
             }
-    __syncthreads();
+    wave_lds_sync();
     float part[4][4];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
@@ -197,7 +207,7 @@ __device__ inline float mlp_forward_z3(const MlpShared &s, const float feat[kMlp
             const float v = __shfl(part[mt][i], src);
             if ((lane >> 4) == mt && (lane & 3) == i) mine = v;
         }
-    __syncthreads();
+    wave_lds_sync();
     return mine + s.b3;
 }
 

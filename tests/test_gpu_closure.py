@@ -358,3 +358,32 @@ def test_trained_closure_on_held_out_pairs(gpu):
         say(f"    n={n} seed={seed}: loss {ev['loss']:.6f} vs 4 for mu_sgs = 0, gain {ev['gain']:+.5f}")
         assert ev["used_model"]
         assert ev["loss"] <= 4.0 * (1 + 1e-3)
+
+
+def test_constant_fit_on_noise_free_pairs(gpu):
+    """The least-squares constant mu_sgs of a pair (closure.constant_fit, from
+    Rd(0) and Rd(1) only) against the loss evaluated through the diffusion
+    kernel at mu_sgs = c*, 0.5 c* and 1.5 c*: L(c) = 4 - 2c num + c^2 den is a
+    parabola with minimum 4 (1 - gain) at c*.  On the noise-free Taylor-Green
+    pairs the optimum is negative (anti-diffusion), so a closure, whose
+    mu_sgs >= 0, cannot gain there: its best is mu_sgs = 0."""
+    from minisod2d_amd import closure as cl
+    say("[gpu] closure: constant fit on a noise-free 8^3 pair (k = 2)")
+    pair = cl.make_pair(8, seed=0, smooth_k=2)
+    c, gain = cl.constant_fit(pair)
+    p = pair.coarse
+    nn = p.tables.nnode
+
+    def loss(cv):
+        mu = np.full((p.nelem, nn), cv, order="F")
+        _, Rd = cl.gpu_residuals(p, mu_sgs=mu)
+        r = cl._mom_ener(Rd[1], Rd[2])
+        return float((pair.weights * ((r - pair.target) ** 2).sum(axis=1)).sum())
+
+    L0, Lc, Llo, Lhi = loss(0.0), loss(c), loss(0.5 * c), loss(1.5 * c)
+    say(f"    c* = {c:.4e}, gain {gain:.4e}; L(0) = {L0:.12f}, L(c*) = {Lc:.12f}")
+    assert abs(L0 - 4.0) <= 1e-9
+    assert abs(Lc - 4.0 * (1.0 - gain)) <= 1e-9
+    # parabola through the three points: L(c/2) = L(3c/2) = L(c*) + gain
+    assert abs(Llo - Lc - gain) <= 1e-9 and abs(Lhi - Lc - gain) <= 1e-9
+    assert c < 0.0 and 0.0 <= gain < 1e-3
