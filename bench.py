@@ -33,8 +33,8 @@ FP64_PEAK = 78.6e12  # MI355X vector FP64, flop/s
 CALIB_BYTES = 2 << 30  # device-to-device copy that scales the HBM byte counters
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_convec_diffu_128cubed_p4.md")
 TRAFFIC_TAG = "traffic-json: "
-KERNEL_RE = {"full_convec_ijk": r"k_convec<(\d+), double, double(, double)?>",
-             "full_diffusion_ijk": r"k_diffu<(\d+), double, double, false(, double)?>",
+KERNEL_RE = {"full_convec_ijk": r"k_convec<(\d+), msod::ConvecArgs<double, double, double, double, double>, double>",
+             "full_diffusion_ijk": r"k_diffu<(\d+), msod::DiffuArgs<double, double, double, double>, false, double>",
              "fem_convec": r"k_fem_convec<true>", "fem_diffu": r"k_fem_diffu<true>",
              "zero_fill": r"msod::k_zero\("}
 
@@ -124,6 +124,54 @@ def summarize_pmc(fetch_csv, write_csv, n, porder):
                 ent["traffic_over_algorithmic"] = ent["traffic_bytes"] / algo[name]
     out["workload"] = {"n": n, "porder": porder, "npoin": prob_n, "elem_nodes": E}
     return out
+
+
+def short_kernel_name(k):
+    """A kernel's demangled name without its argument list, storage types
+    shortened (d = double, f = float)."""
+    import re
+    k = k.replace("(anonymous namespace)::", "")
+    k = re.sub(r"^void ", "", k.split("(")[0]).replace("msod::", "")
+    k = re.sub(r"(Convec|Diffu)Args<([^>]*)>", lambda mo: mo.group(1) + "<" + "".join(
+        t.strip()[0] for t in mo.group(2).split(",")) + ">", k)
+    return k
+
+
+def summarize_counters(paths, min_ms=None):
+    """Per kernel, the mean of every counter per dispatch over one or more
+    rocprofv3 --pmc csv files (one counter set per run), as a markdown table
+    with derived ratios where their counters are present."""
+    import csv
+    acc = {}
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            key = short_kernel_name(r["Kernel_Name"])
+            d = acc.setdefault(key, {})
+            c = d.setdefault(r["Counter_Name"], {})
+            c[r["Dispatch_Id"]] = c.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+    names = sorted({c for d in acc.values() for c in d})
+    ratios = [("wait/wave", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"),
+              ("wait_issue/wave", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
+              ("valu/wave", "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES"),
+              ("lds/wave", "SQ_ACTIVE_INST_LDS", "SQ_WAVE_CYCLES"),
+              ("bank conflict/LDS inst", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_LDS"),
+              ("L2 hit rate", "TCC_HIT_sum", None)]
+    rows = []
+    for k, d in acc.items():
+        mean = {c: sum(v.values()) / len(v) for c, v in d.items()}
+        ndisp = max(len(v) for v in d.values())
+        row = [k, str(ndisp)] + [f"{mean[c]:.4g}" if c in mean else "" for c in names]
+        for _, a, b in ratios:
+            if b is None:
+                row.append(f"{mean[a] / (mean[a] + mean['TCC_MISS_sum']):.3f}"
+                           if a in mean and "TCC_MISS_sum" in mean else "")
+            else:
+                row.append(f"{mean[a] / mean[b]:.3f}" if a in mean and b in mean and mean[b] else "")
+        rows.append(row)
+    head = ["kernel", "dispatches"] + names + [r[0] for r in ratios]
+    out = ["| " + " | ".join(head) + " |", "|" + "---|" * len(head)]
+    out += ["| " + " | ".join(r) + " |" for r in sorted(rows)]
+    return "\n".join(out)
 
 
 def load_traffic(n, porder, kernel):
@@ -594,7 +642,12 @@ def main():
                     help="skip the 48^3 porder 7 precision record and the 96^3 porder 3 closure record")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print HBM bytes per dispatch from two counter runs and exit")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
+                    help="print every counter's mean per dispatch and kernel (markdown) and exit")
     args = ap.parse_args()
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters))
+        return
     if args.summarize_pmc:
         print(json.dumps(summarize_pmc(*args.summarize_pmc, args.n, args.porder)))
         return

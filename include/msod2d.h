@@ -270,9 +270,11 @@ int msod_full_diffusion_ijk_f32split(int porder, int nelem, int npoin, const int
  * full_diffusion_ijk with the reference's full argument lists
  * (elem_convec.f90:198-211, elem_diffu.f90:10-23) in float, Cp and Pr
  * included, and float outputs.  The kernels compute in FP64 from the float
- * inputs and round each node's sum once to float, so the only errors are the
- * float storage of inputs and outputs (the reference at rp = 4 also rounds
- * inside every product and sum). */
+ * inputs and sum each batch's contributions to a node in FP64; each batch's
+ * sum is rounded once to float and added with a float atomic (a node shared
+ * by k batches takes k roundings), so the errors are the float storage of
+ * inputs and outputs and those few additions (the reference at rp = 4
+ * rounds inside every product and sum). */
 int msod_full_convec_ijk_r4(int porder, int nelem, int npoin, const int32_t *connec, const float *Ngp,
                             const float *dNgp, const float *He, const float *gpvol,
                             const float *dlxigp_ip, const float *xgp, const int32_t *invAtoIJK,

@@ -39,8 +39,10 @@ def test_canonical_bytes_128cubed_p4():
 
 def test_summarize_pmc_scales_by_the_copy(tmp_path):
     cb = bench.CALIB_BYTES
-    conv = "void msod::k_convec<4, double, double>(msod::ConvecArgs<double, double>)"
-    diff = "void msod::k_diffu<4, double, double, false>(msod::DiffuArgs<double, double>, float const*)"
+    conv = ("void msod::k_convec<4, msod::ConvecArgs<double, double, double, double, double>, double>"
+            "(msod::ConvecArgs<double, double, double, double, double>)")
+    diff = ("void msod::k_diffu<4, msod::DiffuArgs<double, double, double, double>, false, double>"
+            "(msod::DiffuArgs<double, double, double, double>, float const*)")
     # the fetch counter reads half of what the copy moves; the write counter all of it
     fetch = tmp_path / "fetch.csv"
     write = tmp_path / "write.csv"
@@ -92,3 +94,21 @@ def test_minimal_flops_128cubed_p4():
     blk = bench.flops_block(128 ** 3 * 125, 5, {"convec_ms": 10.0, "diffu_ms": 10.0})
     assert blk["convec_tflops"] == pytest.approx(fc / 1e-2 / 1e12)
     assert blk["pair_fp64_peak_frac"] == pytest.approx((fc + fd) / 2e-2 / bench.FP64_PEAK)
+
+
+def test_summarize_counters_per_kernel(tmp_path):
+    conv = ("void msod::k_convec<4, msod::ConvecArgs<double, double, double, double, double>, double>"
+            "(msod::ConvecArgs<double, double, double, double, double>)")
+    split = ("void msod::k_convec<4, msod::ConvecArgs<float, float, double, double, double>, double>"
+             "(msod::ConvecArgs<float, float, double, double, double>)")
+    a = tmp_path / "a.csv"
+    b = tmp_path / "b.csv"
+    _write_counters(a, "SQ_WAIT_ANY", [(1, conv, 30.0), (2, conv, 50.0), (3, split, 10.0)])
+    _write_counters(b, "SQ_WAVE_CYCLES", [(1, conv, 100.0), (2, conv, 100.0), (3, split, 40.0)])
+    table = bench.summarize_counters([str(a), str(b)]).splitlines()
+    rows = {r.split("|")[1].strip(): [x.strip() for x in r.split("|")[2:-1]] for r in table[2:]}
+    head = [x.strip() for x in table[0].split("|")[2:-1]]
+    c = dict(zip(head, rows["k_convec<4, Convec<ddddd>, double>"]))
+    assert c["dispatches"] == "2" and float(c["SQ_WAIT_ANY"]) == 40.0 and float(c["wait/wave"]) == 0.4
+    f = dict(zip(head, rows["k_convec<4, Convec<ffddd>, double>"]))
+    assert float(f["wait/wave"]) == 0.25
