@@ -18,7 +18,8 @@
 //     column then contract them (eleven accumulators over the three
 //     directions; the sweep code is shared by the directions).  LDS holds the
 //     ten values of a node contiguously, so a source costs one address and
-//     five 16-byte reads.
+//     five 16-byte reads (nine values, rH formed in the sweep, where that
+//     fits another workgroup per CU: convec_drop_rh).
 //     The result equals the reference's algebra exactly in real arithmetic;
 //     rounding differs at the 1e-16 level.
 //  3. residual x gpvol, added into the batch's per-node sums in LDS in a
@@ -59,18 +60,47 @@ __device__ __forceinline__ void convec_fetch(const A &a, int64_t e, int l, uint3
     x.li = __builtin_nontemporal_load(&a.plan.lidx[e * NN + l]);
 }
 
-// Launch bound: 3 workgroups per CU (up to 168 VGPRs), room for the loads in
-// flight; 4 would force spills.  TA: arithmetic type of the gathers, fluxes
-// and sweeps (FP64, or FP32 for the FP32-arithmetic form); the node sums and
-// the assembly stay FP64.
+// LDS bytes of a workgroup with NF values per node record.
+template <int P, typename TA>
+constexpr size_t convec_lds(int nf)
+{
+    using S = CShape<P>;
+    return sizeof(TA) * (S::Q * S::Q + (size_t)S::EPB * S::NN * nf) + sizeof(double) * 5 * S::UMAX +
+           3 * sizeof(uint32_t) * kMaxKpb;
+}
+// rH = rho E + rho k is left out of the records (formed as r H in the sweep)
+// where that fits one more workgroup per CU, for FP32 element arrays with
+// FP64 arithmetic at porder 6 and 7 (p=7: 82.6 -> 78.5 KB, two 512-thread
+// workgroups instead of one, at most 128 VGPRs: 44 B of spills per lane;
+// 48^3 p=7: 3.66 -> 3.35 ms for the FP32 split, 3.51 -> 3.21 ms with every
+// input in FP32).  With FP64 element arrays the 128-VGPR bound spills 108 B
+// per lane and ran slower (3.98 -> 4.18 ms): one workgroup there.
+template <int P, typename TA, typename TG>
+constexpr bool convec_drop_rh()
+{
+    return sizeof(TG) < sizeof(TA) && 163840 / convec_lds<P, TA>(9) > 163840 / convec_lds<P, TA>(10);
+}
+// Launch bound in waves per SIMD: 3 (up to 168 VGPRs, room for the loads in
+// flight; 4 would force spills at p=4); 4 for 512-thread workgroups whose
+// LDS lets two share a CU.
+template <int P, typename TA, typename TG>
+constexpr int convec_waves()
+{
+    return CShape<P>::NT == 512 && convec_drop_rh<P, TA, TG>() ? 4 : 3;
+}
+
+// TA: arithmetic type of the gathers, fluxes and sweeps (FP64, or FP32 for
+// the FP32-arithmetic form); the node sums and the assembly stay FP64.
 template <int P, class A, typename TA = double>
-__global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
+__global__ void __launch_bounds__(CShape<P>::NT, (convec_waves<P, TA, typename A::G>())) k_convec(A a)
 {
     using TG = typename A::G;
     using S = CShape<P>;
     using T = TA;  // arithmetic
     constexpr int Q = S::Q, NN = S::NN, EPB = S::EPB, UMAX = S::UMAX, NPASS = S::NPASS;
-    constexpr int NF = 10;
+    // record: r, p, H, [rH,] u(3), q(3)
+    constexpr bool kRH = !convec_drop_rh<P, TA, TG>();
+    constexpr int NF = kRH ? 10 : 9, iU = kRH ? 4 : 3, iQ = iU + 3;
     __shared__ T sD[Q * Q];
     __shared__ __attribute__((aligned(16))) T sF[EPB][NN][NF];  // node-major: one base address per source
     __shared__ double sR[5][UMAX];
@@ -155,13 +185,13 @@ __global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
                 sF[le][l][0] = r;
                 sF[le][l][1] = pv;
                 sF[le][l][2] = H;
-                sF[le][l][3] = rH;
-                sF[le][l][4] = ux;
-                sF[le][l][5] = uy;
-                sF[le][l][6] = uz;
-                sF[le][l][7] = cur.q[0];
-                sF[le][l][8] = cur.q[1];
-                sF[le][l][9] = cur.q[2];
+                if constexpr (kRH) sF[le][l][3] = rH;
+                sF[le][l][iU] = ux;
+                sF[le][l][iU + 1] = uy;
+                sF[le][l][iU + 2] = uz;
+                sF[le][l][iQ] = cur.q[0];
+                sF[le][l][iQ + 1] = cur.q[1];
+                sF[le][l][iQ + 2] = cur.q[2];
             }
             const uint32_t li = cur.li;
             // issued now, used later: He and gpvol of this pass (after the
@@ -200,17 +230,18 @@ __global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
                     const int Id = d == 0 ? I : (d == 1 ? J : K);
                     const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
                     const int base = l - Id * stride;
-                    const T Ua = h0 * sF[le][l][4] + h1 * sF[le][l][5] + h2 * sF[le][l][6];
-                    const T Qa = h0 * sF[le][l][7] + h1 * sF[le][l][8] + h2 * sF[le][l][9];
+                    const T Ua = h0 * sF[le][l][iU] + h1 * sF[le][l][iU + 1] + h2 * sF[le][l][iU + 2];
+                    const T Qa = h0 * sF[le][l][iQ] + h1 * sF[le][l][iQ + 1] + h2 * sF[le][l][iQ + 2];
                     // 1-D derivatives along this line of the stored values
                     T dp = 0, du[3] = {0, 0, 0}, dq[3] = {0, 0, 0}, dr = 0, dH = 0, drH = 0;
 #pragma unroll 1
                     for (int n = 0; n < Q; ++n) {
                         const int s = base + n * stride;
                         const T Dv = sD[Id * Q + n];
-                        const T rs = sF[le][s][0], ps = sF[le][s][1], Hs = sF[le][s][2], rHs = sF[le][s][3];
-                        const T us0 = sF[le][s][4], us1 = sF[le][s][5], us2 = sF[le][s][6];
-                        const T qs0 = sF[le][s][7], qs1 = sF[le][s][8], qs2 = sF[le][s][9];
+                        const T rs = sF[le][s][0], ps = sF[le][s][1], Hs = sF[le][s][2];
+                        const T rHs = kRH ? sF[le][s][kRH ? 3 : 0] : rs * Hs;
+                        const T us0 = sF[le][s][iU], us1 = sF[le][s][iU + 1], us2 = sF[le][s][iU + 2];
+                        const T qs0 = sF[le][s][iQ], qs1 = sF[le][s][iQ + 1], qs2 = sF[le][s][iQ + 2];
                         dr += Dv * rs;
                         dH += Dv * Hs;
                         drH += Dv * rHs;
@@ -233,9 +264,9 @@ __global__ void __launch_bounds__(CShape<P>::NT, 3) k_convec(A a)
                     aU += h0 * du[0] + h1 * du[1] + h2 * du[2];
                     aQ += h0 * dq[0] + h1 * dq[1] + h2 * dq[2];
                 }
-                const T H = sF[le][l][2], rH = sF[le][l][3];
-                const T uu[3] = {sF[le][l][4], sF[le][l][5], sF[le][l][6]};
-                const T qq[3] = {sF[le][l][7], sF[le][l][8], sF[le][l][9]};
+                const T H = sF[le][l][2], rH = kRH ? sF[le][l][kRH ? 3 : 0] : r * H;
+                const T uu[3] = {sF[le][l][iU], sF[le][l][iU + 1], sF[le][l][iU + 2]};
+                const T qq[3] = {sF[le][l][iQ], sF[le][l][iQ + 1], sF[le][l][iQ + 2]};
                 const T Rm = (T)0.5 * (aQ + r * aU) + (T)0.5 * sRho;
                 const T Re = (T)0.25 * (aE + H * aQ + rH * aU) + (T)0.25 * (sE + H * sRho);
                 T Rmo[3];
