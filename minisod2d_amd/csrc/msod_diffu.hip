@@ -94,11 +94,23 @@ __device__ __forceinline__ void diffu_stage_mu(const A &a, int64_t e0, TG (*sMu)
     }
 }
 
-// The closure form is bounded to 3 workgroups per CU: the compiler then keeps
-// the MFMA accumulators in VGPRs (about 148 in all) instead of 188 VGPRs and
-// 32 AGPRs, 3 waves per SIMD instead of 2 (4.7 against 6.2 ms at 96^3 p=3).
+// Launch bound in waves per SIMD.  The closure form is bounded to 3
+// workgroups per CU: the compiler then keeps the MFMA accumulators in VGPRs
+// (about 148 in all) instead of 188 VGPRs and 32 AGPRs, 3 waves per SIMD
+// instead of 2 (4.7 against 6.2 ms at 96^3 p=3).  512-thread workgroups with
+// FP32 element arrays and FP64 arithmetic: 6, three workgroups per CU (their
+// LDS fits three), at 80 VGPRs with 20 B of spills per lane (48^3 p=7: FP32
+// split 3.96 -> 3.45 ms, every input in FP32 4.21 -> 3.41 ms); the FP64 form
+// spilled 28 B there and ran slower (3.61 -> 3.90 ms), so it keeps two.
+template <int P, typename TG, typename TA, bool CLOSURE>
+constexpr int diffu_waves()
+{
+    return CLOSURE ? (Shape<P>::NT == 256 ? 3 : 1)
+                   : (Shape<P>::NT == 512 && sizeof(TG) < sizeof(TA) ? 6 : 1);
+}
+
 template <int P, class A, bool CLOSURE, typename TA = double>
-__global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ? 3 : 1) k_diffu(A a, const float *mlp)
+__global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, TA, CLOSURE>())) k_diffu(A a, const float *mlp)
 {
     using TG = typename A::G;
     using S = Shape<P>;
@@ -216,9 +228,11 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
             const int64_t en = e + EPB;
             if (le < EPB && en < a.nelem) diffu_fetch<NN, CLOSURE>(a, en, l, nxt);
         }
-        // this pass's element arrays, issued before the barrier (read once:
-        // non-temporal, spares L2); unconditional, idle threads read a real
-        // element; kept in the storage type until used
+        // this pass's element arrays, issued before the barrier;
+        // unconditional, idle threads read a real element; kept in the
+        // storage type until used.  Plain loads: with the non-temporal hint
+        // the 128^3 p=4 diffusion ran 4% slower in FP64 and 5-7% with FP32
+        // element arrays (the hint still pays in the convection)
         TG hr[3][3], gvr;
         uint32_t li;
         {
@@ -227,9 +241,9 @@ __global__ void __launch_bounds__(Shape<P>::NT, CLOSURE && Shape<P>::NT == 256 ?
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) hr[i][d] = __builtin_nontemporal_load(&geo[(i + 3 * d) * NN]);
-            gvr = __builtin_nontemporal_load(&a.gvol[ec * NN + l]);
-            li = __builtin_nontemporal_load(&pl.lidx[ec * NN + l]);
+                for (int i = 0; i < 3; ++i) hr[i][d] = geo[(i + 3 * d) * NN];
+            gvr = a.gvol[ec * NN + l];
+            li = pl.lidx[ec * NN + l];
         }
         __syncthreads();
 
