@@ -147,7 +147,7 @@ def test_full_size_sampled_parity_and_uniform_state(gpu, n, p, jitter, jump):
 # ---------------------------------------------------------------------------
 SPLIT_F32 = ("He", "gpvol", "q", "rho", "E", "Tem", "mu_fluid", "mu_e", "mu_sgs")   # u, pr stay FP64
 ALL_F32 = SPLIT_F32 + ("u", "pr")
-BAR = 1e-6   # north-star bar of the FP32 split against the FP64 oracle
+BAR = 1e-6   # accuracy bar of the FP32 split against the FP64 oracle (relative L2)
 
 
 def rounded(prob, fields):

@@ -177,7 +177,7 @@ def test_fp32_arithmetic_error(gpu, p, n):
     assert max(kern.values()) > 1e-10  # the sweeps really ran in FP32
 
 
-TOL_SPLIT = 1e-6  # north-star bar of the FP32 split against the FP64 oracle
+TOL_SPLIT = 1e-6  # accuracy bar of the FP32 split against the FP64 oracle (relative L2)
 
 
 @pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (7, 3)])
