@@ -184,16 +184,16 @@ def pair_seeds(seed: int):
 
 
 def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float = 0.0,
-              log=None, smooth_k: int = 0) -> ClosurePair:
+              log=None, smooth_k: int = 0, smooth_mu_e: float = 5e-4) -> ClosurePair:
     """Training pair on n^3 affine hexahedra (see the module docstring);
     smooth_k > 0: the noise-free fields of smooth_fields at wavenumber
-    smooth_k instead of the synthetic ones."""
+    smooth_k (with mu_e = smooth_mu_e) instead of the synthetic ones."""
     t0 = time.time()
     say = log or (lambda *_: None)
     seeds = pair_seeds(seed)
     fine = box_problem(n, pf, jitter=0.0, rho_jump=rho_jump, seeds=seeds)
     if smooth_k:
-        smooth_fields(fine, smooth_k)
+        smooth_fields(fine, smooth_k, mu_e=smooth_mu_e)
     say(f"  n={n}: order-{pf} problem built ({time.time() - t0:.1f} s)")
     fine.mu_sgs[:] = 0.0
     Rcf, Rdf = gpu_residuals(fine)
@@ -220,6 +220,7 @@ def make_pair(n: int, seed: int = 0, pf: int = 7, pt: int = 3, rho_jump: float =
     gap = ((r0 - target) ** 2).sum(axis=1)
     weights = 1.0 / np.maximum(gap, np.finfo(float).tiny)
     meta = dict(n=n, seed=seed, seeds=list(seeds), pf=pf, pt=pt, rho_jump=rho_jump, smooth_k=smooth_k,
+                smooth_mu_e=smooth_mu_e if smooth_k else None,
                 nelem=coarse.nelem, npoin=coarse.npoin, build_s=round(time.time() - t0, 2),
                 fine_norm=np.linalg.norm(_mom_ener(rt[:, 0:3], rt[:, 3]), axis=1).tolist())
     if log:
@@ -516,19 +517,20 @@ def constant_fit(pair: ClosurePair) -> tuple[float, float]:
 
 
 def diagnose(k: int, sizes, val_sizes, held_out, steps: int = 300, lr: float = 3e-3, seed: int = 0,
-             log=None) -> dict:
+             log=None, mu_e: float = 5e-4) -> dict:
     """Noise-free pairs (smooth_fields at wavenumber k): the least-squares
     constant mu_sgs and its gain on every pair, then a network trained on the
     pairs of `sizes` (early stopping on `val_sizes`) and its gain on the
     held-out sizes, which it never saw."""
-    mk = lambda ns, off: [make_pair(n, seed=seed + off + i, smooth_k=k, log=log) for i, n in enumerate(ns)]
+    mk = lambda ns, off: [make_pair(n, seed=seed + off + i, smooth_k=k, smooth_mu_e=mu_e, log=log)
+                          for i, n in enumerate(ns)]
     groups = {"train": mk(sizes, 0), "val": mk(val_sizes, 100), "held_out": mk(held_out, 200)}
     const = {g: [dict(n=p.meta["n"], c=c, gain=gn) for p in ps for c, gn in [constant_fit(p)]]
              for g, ps in groups.items()}
     res = train(groups["train"], steps=steps, lr=lr, seed=seed, log=log, val_pairs=groups["val"] or None)
     gain = {g: [dict(n=p.meta["n"], gain=evaluate(p, res["params"])["gain"]) for p in ps]
             for g, ps in groups.items()}
-    return dict(k=k, constant=const, model=gain, best_step=res.get("best_step"),
+    return dict(k=k, mu_e=mu_e, constant=const, model=gain, best_step=res.get("best_step"),
                 loss_first=res["history"][0], loss_last=res["history"][-1], params=res["params"],
                 adam_m=res["adam_m"], adam_v=res["adam_v"], step=res["step"])
 
@@ -610,6 +612,7 @@ def _main(argv=None) -> int:
     dg.add_argument("--steps", type=int, default=300)
     dg.add_argument("--lr", type=float, default=3e-3)
     dg.add_argument("--seed", type=int, default=0)
+    dg.add_argument("--mu-e", type=float, default=5e-4, help="constant mu_e of the noise-free fields")
     dg.add_argument("--out", default=None, help="weight file for the model of the last k")
     ev = sub.add_parser("evaluate")
     ev.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -636,7 +639,8 @@ def _main(argv=None) -> int:
         print(json.dumps(dict(out=a.out, sha256=sha, **meta)), flush=True)
     elif a.cmd == "diagnose":
         for k in a.k:
-            r = diagnose(k, a.sizes, a.val_sizes, a.held_out, steps=a.steps, lr=a.lr, seed=a.seed, log=log)
+            r = diagnose(k, a.sizes, a.val_sizes, a.held_out, steps=a.steps, lr=a.lr, seed=a.seed, log=log,
+                         mu_e=a.mu_e)
             if a.out and k == a.k[-1]:
                 meta = dict(smooth_k=k, sizes=a.sizes, val_sizes=a.val_sizes, held_out=a.held_out,
                             steps=r["step"], lr=a.lr, constant=r["constant"], model=r["model"])
