@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(CShape<P>::NT, (convec_waves<P, TA, typename A
     };
 
     for (int i = t; i < nbat; i += blockDim.x) {
-        const uint32_t bi = pl.batches[first + i];
+        const uint32_t bi = plan_batch(pl.batches, first + i);
         sB[i] = bi;
         sU0[i] = pl.uoff[bi];
         sU1[i] = pl.uoff[bi + 1];
@@ -142,10 +142,10 @@ __global__ void __launch_bounds__(CShape<P>::NT, (convec_waves<P, TA, typename A
     ConvecNode<A> nd[2];
     uint32_t gnext;  // global node of the next pass's element node
     {
-        const int64_t b0 = pl.batches[first];
+        const int64_t b0 = plan_batch(pl.batches, first);
         const int64_t e0 = elem(b0, 0);
         convec_fetch<NN>(a, e0, l, __builtin_nontemporal_load(&pl.conn[e0 * NN + l]), nd[0]);
-        const int64_t b1 = npass > 1 ? (int64_t)pl.batches[first + 1 / NPASS] : b0;
+        const int64_t b1 = npass > 1 ? (int64_t)plan_batch(pl.batches, first + 1 / NPASS) : b0;
         const int64_t e1 = npass > 1 ? elem(b1, 1 % NPASS) : e0;
         gnext = __builtin_nontemporal_load(&pl.conn[e1 * NN + l]);
     }

@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
 
     const Plan &pl = a.plan;
     constexpr int NT = S::NT, UPT = (UMAX + NT - 1) / NT;
-    const int64_t b = pl.batches[xcd_slot(blockIdx.x, gridDim.x)];
+    const int64_t b = plan_batch(pl.batches, xcd_slot(blockIdx.x, gridDim.x));
     const uint32_t u0 = pl.uoff[b], U = pl.uoff[b + 1] - u0;
     const int t = threadIdx.x;
     uint32_t wn[UPT];  // this thread's entries of the batch's node list, for the write phase

@@ -21,7 +21,8 @@
 //                       when this batch is the first writer of the node
 //   batches[nb]         the batches processed by one launch; a workgroup takes
 //                       kpb consecutive ones, one after the other, with the
-//                       loads of the next pass always in flight
+//                       loads of the next pass always in flight.  Null: the
+//                       batches 0 .. nb-1 in mesh order, with no list to load
 // Within a batch the element-node values are summed in a fixed order
 // (add_pass).  Two ways to combine the batches' partial sums at nodes shared
 // between batches (mode):
@@ -177,6 +178,12 @@ hipError_t launch_zero_nodes(TO *Rmass, TO *Rmom, TO *Rener, int64_t npoin, cons
 template <typename T>
 hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
                             const int32_t *l2g, T *geo, T *gvol, hipStream_t s);
+
+// Entry i of a launch's batch list (Plan::batches); a null list is mesh order.
+__device__ __forceinline__ uint32_t plan_batch(const uint32_t *batches, uint32_t i)
+{
+    return batches ? batches[i] : i;
+}
 
 // Blocks of a launch are dealt to the chip's 8 XCDs round robin (block i to
 // XCD i % 8), each XCD with its own L2.  xcd_slot gives every XCD a

@@ -297,7 +297,6 @@ struct PlanEntry {
     int B = 1;
     int64_t nbatch = 0;
     uint32_t *conn = nullptr, *uoff = nullptr, *unode = nullptr, *order = nullptr;
-    uint32_t *natural = nullptr;  // the batches in mesh order
     uint16_t *lidx = nullptr;
     std::vector<int64_t> color_off;  // batches of colour c: order[color_off[c] .. color_off[c+1])
     uint32_t *unref = nullptr;       // nodes no element references (zeroed per call)
@@ -306,9 +305,9 @@ struct PlanEntry {
     void release()
     {
         for (void *p : {(void *)conn, (void *)uoff, (void *)unode, (void *)order, (void *)lidx,
-                        (void *)unref, (void *)natural})
+                        (void *)unref})
             if (p) (void)hipFree(p);
-        conn = uoff = unode = order = unref = natural = nullptr;
+        conn = uoff = unode = order = unref = nullptr;
         lidx = nullptr;
     }
 };
@@ -602,11 +601,6 @@ int build_plan(const void *dconnec, int64_t nelem, int64_t npoin, const TableSet
     if ((st = upload(H.uoff, &P.uoff, "plan upload"))) return st;
     if ((st = upload(H.unode, &P.unode, "plan upload"))) return st;
     if ((st = upload(H.order, &P.order, "plan upload"))) return st;
-    {
-        std::vector<uint32_t> nat(H.nbatch);
-        for (int64_t b = 0; b < H.nbatch; ++b) nat[b] = (uint32_t)b;
-        if ((st = upload(nat, &P.natural, "plan upload"))) return st;
-    }
     if ((st = upload(H.unref, &P.unref, "plan upload"))) return st;
     P.nunref = (int64_t)H.unref.size();
     return MSOD_OK;
@@ -868,7 +862,7 @@ int run_assembled(A &a, const PlanEntry &P, int timer, L launch, const char *wha
     }
     TimedLaunch tk(timer);
     if (g.assembly == kAtomic) {
-        pl.batches = natural ? P.natural : P.order;
+        pl.batches = natural ? nullptr : P.order;  // null: mesh order, no list
         pl.nb = P.nbatch;
         pl.kpb = kpb_for(pl.nb);
         a.plan = pl;
