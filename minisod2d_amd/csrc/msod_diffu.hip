@@ -44,10 +44,10 @@ struct DiffuNode {
     typename A::U u[3];
 };
 
-template <int NN, bool CLOSURE, class A>
-__device__ __forceinline__ void diffu_fetch(const A &a, int64_t e, int l, DiffuNode<A> &x)
+template <class A>
+__device__ __forceinline__ void diffu_fetch(const A &a, uint32_t gnode, DiffuNode<A> &x)
 {
-    const int64_t g = __builtin_nontemporal_load(&a.plan.conn[e * NN + l]);
+    const int64_t g = gnode;
     const int64_t np = a.npoin;
     x.r = a.rho[g];
     x.T_ = a.Tem[g];
@@ -159,6 +159,14 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
     const T Cp = (T)a.Cp, Pr = (T)a.Pr;
 
     DiffuNode<A> cur, nxt;
+    // the connectivity is read a pass ahead of the gathers that use it, so
+    // that the next pass's gathers issue without a trip to memory (idle
+    // threads and elements past the mesh read a real entry)
+    uint32_t gnext = 0;
+    auto conn_of = [&](int p) -> uint32_t {
+        const int64_t ep = b * S::BATCH + p * EPB + (le < EPB ? le : 0);
+        return __builtin_nontemporal_load(&pl.conn[(ep < a.nelem ? ep : a.nelem - 1) * NN + l]);
+    };
     static_assert(Q * Q <= S::NT, "one derivative-table entry per thread");
     {
         // prologue: the tables and the connectivity first, then the values
@@ -178,7 +186,9 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
         }
         const int64_t e = b * S::BATCH + le;
         const int64_t ec = e < a.nelem ? e : a.nelem - 1;
-        diffu_fetch<NN, CLOSURE>(a, ec, l, cur);
+        const uint32_t gcur = __builtin_nontemporal_load(&pl.conn[ec * NN + l]);
+        if (S::NPASS > 1) gnext = conn_of(1);
+        diffu_fetch(a, gcur, cur);
         if (!kMuPerPass) {
 #pragma unroll
             for (int k = 0; k < IT; ++k) {
@@ -226,7 +236,8 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
         }
         if (pass + 1 < S::NPASS) {  // next pass's inputs: in flight during this pass
             const int64_t en = e + EPB;
-            if (le < EPB && en < a.nelem) diffu_fetch<NN, CLOSURE>(a, en, l, nxt);
+            if (le < EPB && en < a.nelem) diffu_fetch(a, gnext, nxt);
+            if (pass + 2 < S::NPASS) gnext = conn_of(pass + 2);
         }
         // this pass's element arrays, issued before the barrier;
         // unconditional, idle threads read a real element; kept in the
