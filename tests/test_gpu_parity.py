@@ -90,6 +90,35 @@ def test_fp64_matches_oracle_all_porders(gpu, p):
         assert rel_l2(got, want) <= TOL64, (key, rel_l2(got, want))
 
 
+@pytest.mark.parametrize("p", range(1, 9))
+def test_fp64_ragged_last_batch_all_porders(gpu, p):
+    """15 elements: the last batch is short at every porder (p=4: three
+    elements, its second pass holding one; p=3: seven, the second pass three
+    of four; p>=5: the convection's last two-element batch holds one)."""
+    say(f"[gpu] porder {p} jittered 5x3x1 box (ragged last batch) vs oracle")
+    prob = m.box_problem(5, p, ny=3, nz=1, jitter=0.05, rho_jump=0.1, seeds=(31, 32, 33, 34))
+    assert prob.nelem == 15
+    o = Oracle()
+    ref = o.convec(prob) + o.diffu(prob)
+    out = gpu_residuals(prob)
+    for key, got, want in zip(KEYS, out, ref):
+        assert rel_l2(got, want) <= TOL64, (key, rel_l2(got, want))
+
+
+def test_fp64_ragged_workgroup_tail(gpu):
+    """37x31x17 elements at p=4: 4875 batches, the last of three elements;
+    the convection takes two batches per workgroup there (at least 2048
+    workgroups), so its last workgroup holds one batch."""
+    say("[gpu] 37x31x17 p=4 jittered (ragged workgroup tail) vs oracle")
+    prob = m.box_problem(37, 4, ny=31, nz=17, jitter=0.05, threads=min(16, os.cpu_count() or 1))
+    assert prob.nelem == 19499 and -(-prob.nelem // 4) == 4875
+    o = Oracle()
+    ref = o.convec(prob) + o.diffu(prob)
+    out = gpu_residuals(prob)
+    for key, got, want in zip(KEYS, out, ref):
+        assert rel_l2(got, want) <= TOL64, (key, rel_l2(got, want))
+
+
 def test_fp64_config1_64cubed_p4_jittered(gpu):
     """64^3 elements, porder 4, jittered interior nodes (~17M nodes)."""
     say("[gpu] 64^3 p=4 jittered: generating")
