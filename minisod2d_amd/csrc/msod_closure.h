@@ -11,7 +11,7 @@ namespace msod {
 struct ClosureElemArgs {
     int64_t nelem, npoin;
     const uint32_t *conn;      // [nelem][nn] 0-based nodes
-    const double *geo, *gvol;  // element-major geometry
+    const double *geo;  // element-major geometry (GeoNode pairs)
     const double *rho, *u, *Tem;
     double Cp;
     const double *rerr;        // [4][npoin]: momentum x, y, z, energy

@@ -65,12 +65,12 @@ __global__ void __launch_bounds__(Shape<P>::NT) k_closure_elem(ClosureElemArgs a
     double h[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     double gT[3] = {0, 0, 0}, V[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, gv = 0;
     if (active) {
-        const double *geo = a.geo + e * 9 * NN + l;
+        const GeoNode<double> gn = load_geo<NN, false>(a.geo, e, l);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) h[i][d] = geo[(i + 3 * d) * NN];
-        gv = a.gvol[e * NN + l];
+            for (int i = 0; i < 3; ++i) h[i][d] = gn.he(i + 3 * d);
+        gv = gn.gvol();
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int Id = d == 0 ? I : (d == 1 ? J : K);

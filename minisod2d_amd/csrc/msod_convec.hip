@@ -202,11 +202,7 @@ __global__ void __launch_bounds__(CShape<P>::NT, (convec_waves<P, TA, typename A
             const int64_t ec = elem(b, p);
             // kept in the storage type until used: a conversion here would
             // wait for the load before the barrier
-            const TG *geo = a.geo + ec * 9 * NN + l;
-            TG hh[9];
-#pragma unroll
-            for (int c = 0; c < 9; ++c) hh[c] = __builtin_nontemporal_load(&geo[c * NN]);
-            const TG gvr = __builtin_nontemporal_load(&a.gvol[ec * NN + l]);
+            const GeoNode<TG> gn = load_geo<NN, true>(a.geo, ec, l);  // He(i,d) = gn.he(i + 3d)
             convec_fetch<NN>(a, elem_of(g + 1), l, gnext, nxt);
             gnext = __builtin_nontemporal_load(&pl.conn[elem_of(g + 2) * NN + l]);
             __syncthreads();
@@ -224,9 +220,9 @@ __global__ void __launch_bounds__(CShape<P>::NT, (convec_waves<P, TA, typename A
 #pragma unroll 1
                 for (int d = 0; d < 3; ++d) {
                     // one copy of the sweep code for the three directions
-                    const T h0 = (T)(d == 0 ? hh[0] : (d == 1 ? hh[3] : hh[6]));
-                    const T h1 = (T)(d == 0 ? hh[1] : (d == 1 ? hh[4] : hh[7]));
-                    const T h2 = (T)(d == 0 ? hh[2] : (d == 1 ? hh[5] : hh[8]));
+                    const T h0 = (T)(d == 0 ? gn.he(0) : (d == 1 ? gn.he(3) : gn.he(6)));
+                    const T h1 = (T)(d == 0 ? gn.he(1) : (d == 1 ? gn.he(4) : gn.he(7)));
+                    const T h2 = (T)(d == 0 ? gn.he(2) : (d == 1 ? gn.he(5) : gn.he(8)));
                     const int Id = d == 0 ? I : (d == 1 ? J : K);
                     const int stride = d == 0 ? 1 : (d == 1 ? Q : Q * Q);
                     const int base = l - Id * stride;
@@ -273,7 +269,7 @@ __global__ void __launch_bounds__(CShape<P>::NT, (convec_waves<P, TA, typename A
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     Rmo[i] = (T)0.25 * (aM[i] + uu[i] * aQ + qq[i] * aU + uu[i] * sRho + mM[i]);
-                const T gv = (T)gvr;
+                const T gv = (T)gn.gvol();
                 v[0] = (double)(gv * Rm);
                 v[1] = (double)(gv * Rmo[0]);
                 v[2] = (double)(gv * Rmo[1]);

@@ -244,22 +244,17 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
         // storage type until used.  Plain loads: with the non-temporal hint
         // the 128^3 p=4 diffusion ran 4% slower in FP64 and 5-7% with FP32
         // element arrays (the hint still pays in the convection)
-        TG hr[3][3], gvr;
+        GeoNode<TG> gn;  // He(i,d) = gn.he(i + 3d), gpvol = gn.gvol()
         uint32_t li;
         {
             const int64_t ec = e < a.nelem ? (le < EPB ? e : b * S::BATCH + pass * EPB) : a.nelem - 1;
-            const TG *geo = a.geo + ec * 9 * NN + l;
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) hr[i][d] = geo[(i + 3 * d) * NN];
-            gvr = a.gvol[ec * NN + l];
+            gn = load_geo<NN, false>(a.geo, ec, l);
             li = pl.lidx[ec * NN + l];
         }
         __syncthreads();
 
         // ---- pass 1: gradients, stress ------------------------------------
-        // He(i,d) = hr[i][d], converted where used: a conversion placed
+        // He(i,d) = gn.he(i + 3d), converted where used: a conversion placed
         // ahead of the sweeps would wait for every load there
         T gR[3] = {0, 0, 0}, gT[3] = {0, 0, 0}, tau[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         T tauU[3] = {0, 0, 0}, gv = 0, G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -283,7 +278,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
                 }
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const T hid = (T)hr[i][d];
+                    const T hid = (T)gn.he(i + 3 * d);
                     gR[i] += hid * aR;
                     gT[i] += hid * aT;
                     // G[i][j] = d u_i / d x_j = sum_d He(j,d) D_d[u_i]
@@ -296,9 +291,14 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
         if constexpr (CLOSURE) {
             __syncthreads();  // every pass-1 sweep over sN done: the MLP scratch overlays it
             float f[kMlpIn];
-            if (active)
-                mlp_features<T>(G, r, uu, hr, f);
-            else
+            if (active) {
+                T hinv[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) hinv[i][d] = (T)gn.he(i + 3 * d);
+                mlp_features<T>(G, r, uu, hinv, f);
+            } else
 #pragma unroll
                 for (int k = 0; k < kMlpIn; ++k) f[k] = smlp.mean[k];
             const float z3 = mlp_forward_z3(smlp, f, reinterpret_cast<short *>(sbuf) + (t >> 6) * 64 * kMlpH);
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
                 tauU[i] = acc - lam * uu[i];
                 tau[i][i] -= lam;
             }
-            gv = (T)gvr;
+            gv = (T)gn.gvol();
         }
 
         // ---- pass 2: transposed sweeps, one reference direction at a time -
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(Shape<P>::NT, (diffu_waves<P, typename A::G, T
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (active) {
-                const T c0 = gv * (T)hr[0][d], c1 = gv * (T)hr[1][d], c2 = gv * (T)hr[2][d];
+                const T c0 = gv * (T)gn.he(3 * d), c1 = gv * (T)gn.he(1 + 3 * d), c2 = gv * (T)gn.he(2 + 3 * d);
                 sN[le][l][0] = V2{c0 * gR[0] + c1 * gR[1] + c2 * gR[2], c0 * tauU[0] + c1 * tauU[1] + c2 * tauU[2]};
                 sN[le][l][1] = V2{c0 * gT[0] + c1 * gT[1] + c2 * gT[2], c0 * tau[0][0] + c1 * tau[0][1] + c2 * tau[0][2]};
                 sN[le][l][2] = V2{c0 * tau[1][0] + c1 * tau[1][1] + c2 * tau[1][2], c0 * tau[2][0] + c1 * tau[2][1] + c2 * tau[2][2]};

@@ -4,8 +4,11 @@
 // Element data are element-major, with the element-local node index l
 // lexicographic in the reference's 1-D SOD order: l = I + q*(J + q*K),
 // (I,J,K) = (AtoI,AtoJ,AtoK) - 1.
-//   geo   [nelem][9][nn]   He(i,k) at component c = i + 3k (SoA per element)
-//   gvol  [nelem][nn]      gpvol
+//   geo   [nelem][5][nn][2] He(i,k) (component c = i + 3k) and gpvol of each
+//                          element node as five pairs (c = 0,1 | 2,3 | 4,5 |
+//                          6,7 | 8 and gpvol): a node's ten values in five
+//                          16-byte (FP64) or 8-byte (FP32) loads, each wave's
+//                          load contiguous (GeoNode)
 // tab.D[q*q] is the 1-D derivative D[i*q+n] = l_n'(xi_i); tab.l2g[nn] maps l
 // to the 0-based reference (Gmsh) node number, used for arrays kept in the
 // reference layout (mu_e, mu_sgs).
@@ -98,8 +101,8 @@ struct Plan {
 // Workgroups of a launch: each takes kpb consecutive batches of the list.
 inline unsigned plan_blocks(const Plan &p) { return (unsigned)((p.nb + p.kpb - 1) / p.kpb); }
 
-// Kernel arguments.  Storage types: TG for the element-sized arrays (geo,
-// gvol; in the diffusion also mu_e, mu_sgs), TF for the nodal fields.  The
+// Kernel arguments.  Storage types: TG for the element-sized arrays (geo;
+// in the diffusion also mu_e, mu_sgs), TF for the nodal fields.  The
 // kernels compute in FP64 whatever the storage: on gfx950 an FP64 FMA issues
 // at the rate of an unpacked FP32 one, so FP32 storage buys bytes, not time
 // in the ALUs, and FP64 arithmetic keeps rounding out of the derivative
@@ -123,7 +126,7 @@ struct ConvecArgs {
     using Pt = TP;
     int64_t nelem, npoin;
     Plan plan;
-    const TG *geo, *gvol;
+    const TG *geo;
     const TU *u;
     const TF *q, *rho, *E;
     const TP *pr;                    // nullptr: recover pressure from E
@@ -140,7 +143,7 @@ struct DiffuArgs {
     using U = TU;
     int64_t nelem, npoin;
     Plan plan;
-    const TG *geo, *gvol;
+    const TG *geo;
     const TF *rho, *Tem, *mu_fluid;
     const TU *u;
     const TG *mu_e, *mu_sgs;
@@ -177,7 +180,31 @@ hipError_t launch_zero_nodes(TO *Rmass, TO *Rmom, TO *Rener, int64_t npoin, cons
 // Element-major geometry from the reference layouts (see top of file).
 template <typename T>
 hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
-                            const int32_t *l2g, T *geo, T *gvol, hipStream_t s);
+                            const int32_t *l2g, T *geo, hipStream_t s);
+
+// He and gpvol of one element node from geo: He(i,k) = he(i + 3k), with the
+// component index a compile-time constant (a register of the loaded pair, no
+// copy: the values stay in flight until used).
+template <typename T>
+using GeoPair = T __attribute__((ext_vector_type(2)));
+
+template <typename T>
+struct GeoNode {
+    GeoPair<T> v[5];
+    __device__ __forceinline__ T he(int c) const { return (c & 1) ? v[c >> 1].y : v[c >> 1].x; }
+    __device__ __forceinline__ T gvol() const { return v[4].y; }
+};
+
+// NT: non-temporal loads (the geometry is read once per call).
+template <int NN, bool NT, typename T>
+__device__ __forceinline__ GeoNode<T> load_geo(const T *geo, int64_t e, int l)
+{
+    const GeoPair<T> *p = reinterpret_cast<const GeoPair<T> *>(geo) + e * 5 * NN + l;
+    GeoNode<T> g;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) g.v[k] = NT ? __builtin_nontemporal_load(&p[k * NN]) : p[k * NN];
+    return g;
+}
 
 // Entry i of a launch's batch list (Plan::batches); a null list is mesh order.
 __device__ __forceinline__ uint32_t plan_batch(const uint32_t *batches, uint32_t i)

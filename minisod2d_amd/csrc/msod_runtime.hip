@@ -131,9 +131,11 @@ __global__ void k_zero_nodes(TO *Rmass, TO *Rmom, TO *Rener, int64_t npoin,
     }
 }
 
+// geo[e][k][l][0..1]: the pairs (He c = 2k, 2k+1) for k < 4, (He c = 8,
+// gpvol) for k = 4 (GeoNode), from He(3,3,nn,nelem) and gpvol(1,nn,nelem).
 template <typename T>
 __global__ void k_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn, const int32_t *l2g,
-                            T *geo, T *gvol)
+                            T *geo)
 {
     const int64_t total = nelem * nn;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -142,9 +144,16 @@ __global__ void k_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn, 
         const int l = (int)(i - e * nn);
         const int64_t a = l2g[l];
         const T *src = He + 9 * (a + (int64_t)nn * e);
+        T v[10];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) geo[(e * 9 + c) * nn + l] = src[c];
-        gvol[i] = gpvol[a + (int64_t)nn * e];
+        for (int c = 0; c < 9; ++c) v[c] = src[c];
+        v[9] = gpvol[a + (int64_t)nn * e];
+        T *dst = geo + 2 * (e * 5 * nn + l);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            dst[2 * k * nn] = v[2 * k];
+            dst[2 * k * nn + 1] = v[2 * k + 1];
+        }
     }
 }
 
@@ -248,16 +257,16 @@ int convec_batch_size(int porder)
 
 template <typename T>
 hipError_t launch_build_geo(const T *He, const T *gpvol, int64_t nelem, int nn,
-                            const int32_t *l2g, T *geo, T *gvol, hipStream_t s)
+                            const int32_t *l2g, T *geo, hipStream_t s)
 {
     hipLaunchKernelGGL((k_build_geo<T>), dim3(grid_for(nelem * nn, 256)), dim3(256), 0, s, He,
-                       gpvol, nelem, nn, l2g, geo, gvol);
+                       gpvol, nelem, nn, l2g, geo);
     return hipGetLastError();
 }
 template hipError_t launch_build_geo<double>(const double *, const double *, int64_t, int,
-                                             const int32_t *, double *, double *, hipStream_t);
+                                             const int32_t *, double *, hipStream_t);
 template hipError_t launch_build_geo<float>(const float *, const float *, int64_t, int,
-                                            const int32_t *, float *, float *, hipStream_t);
+                                            const int32_t *, float *, hipStream_t);
 
 }  // namespace msod
 
@@ -319,7 +328,7 @@ struct DerivedGeo {
     int porder;
     size_t elt;
     std::vector<int32_t> l2g;
-    void *geo, *gvol;
+    void *geo;  // He and gpvol, element-major pairs (k_build_geo)
 };
 
 // A tetrahedral connectivity found valid (every entry in 1..npoin), with
@@ -624,7 +633,6 @@ void drop_unversioned(const void *src)
         if ((g.geos[i].srcHe == src && g.geos[i].verHe == 0) ||
             (g.geos[i].srcVol == src && g.geos[i].verVol == 0)) {
             (void)hipFree(g.geos[i].geo);
-            (void)hipFree(g.geos[i].gvol);
             g.geos.erase(g.geos.begin() + i);
         }
 }
@@ -662,7 +670,7 @@ int get_plan(const void *dconnec, uint64_t ver, int64_t nelem, int64_t npoin, co
 
 template <typename T>
 int get_geo(const void *dHe, uint64_t vHe, const void *dVol, uint64_t vVol, int64_t nelem,
-            const TableSet *ts, const T **geo, const T **gvol)
+            const TableSet *ts, const T **geo)
 {
     if (vHe == 0) drop_unversioned(dHe);
     if (vVol == 0) drop_unversioned(dVol);
@@ -670,33 +678,24 @@ int get_geo(const void *dHe, uint64_t vHe, const void *dVol, uint64_t vVol, int6
         if (c.srcHe == dHe && c.srcVol == dVol && c.verHe == vHe && c.verVol == vVol &&
             c.nelem == nelem && c.porder == ts->porder && c.elt == sizeof(T) && c.l2g == ts->l2g) {
             *geo = (const T *)c.geo;
-            *gvol = (const T *)c.gvol;
             return MSOD_OK;
         }
     const int nn = (int)ts->l2g.size();
-    DerivedGeo c{dHe, dVol, vHe, vVol, nelem, ts->porder, sizeof(T), ts->l2g, nullptr, nullptr};
-    HIPCHK(hipMalloc(&c.geo, sizeof(T) * 9 * nelem * nn), "hipMalloc element-major He");
-    hipError_t e = hipMalloc(&c.gvol, sizeof(T) * nelem * nn);
-    if (e != hipSuccess) {
-        (void)hipFree(c.geo);
-        return hipfail(e, "hipMalloc element-major gpvol");
-    }
-    e = launch_build_geo<T>((const T *)dHe, (const T *)dVol, nelem, nn, ts->dl2g,
-                                       (T *)c.geo, (T *)c.gvol, g.stream);
+    DerivedGeo c{dHe, dVol, vHe, vVol, nelem, ts->porder, sizeof(T), ts->l2g, nullptr};
+    HIPCHK(hipMalloc(&c.geo, sizeof(T) * 10 * nelem * nn), "hipMalloc element-major He and gpvol");
+    hipError_t e = launch_build_geo<T>((const T *)dHe, (const T *)dVol, nelem, nn, ts->dl2g, (T *)c.geo,
+                                       g.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
     if (e != hipSuccess) {
         (void)hipFree(c.geo);
-        (void)hipFree(c.gvol);
         return hipfail(e, "building element-major geometry");
     }
     if (g.geos.size() >= kMaxDerived) {
         (void)hipFree(g.geos.front().geo);
-        (void)hipFree(g.geos.front().gvol);
         g.geos.erase(g.geos.begin());
     }
     g.geos.push_back(c);
     *geo = (const T *)c.geo;
-    *gvol = (const T *)c.gvol;
     return MSOD_OK;
 }
 
@@ -716,7 +715,6 @@ void drop_derived_in(uintptr_t lo, uintptr_t hi)
     for (size_t i = g.geos.size(); i-- > 0;)
         if (inside(g.geos[i].srcHe) || inside(g.geos[i].srcVol)) {
             (void)hipFree(g.geos[i].geo);
-            (void)hipFree(g.geos[i].gvol);
             g.geos.erase(g.geos.begin() + i);
         }
 }
@@ -910,15 +908,14 @@ int convec_common(int porder, int nelem, int npoin, const int32_t *connec, const
     if ((st = resolve(Rmom, sizeof(TO) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(TO) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
-    const TG *geo, *gvol;
+    const TG *geo;
     if ((st = get_plan(dc, vc, NE, NP, ts, convec_batch_size(porder), &plan))) return st;
-    if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+    if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo))) return st;
 
     ConvecArgs<TG, TF, TO, TU, TP> a;
     a.nelem = NE;
     a.npoin = NP;
     a.geo = geo;
-    a.gvol = gvol;
     a.u = (const TU *)du;
     a.q = (const TF *)dq;
     a.rho = (const TF *)dr;
@@ -971,15 +968,14 @@ int diffu_common(int porder, int nelem, int npoin, const int32_t *connec, const 
     if ((st = resolve(Rmom, sizeof(TO) * 3 * NP, "Rmom", &dRmo, &vx))) return st;
     if ((st = resolve(Rener, sizeof(TO) * NP, "Rener", &dRe, &vx))) return st;
     const PlanEntry *plan;
-    const TG *geo, *gvol;
+    const TG *geo;
     if ((st = get_plan(dc, vc, NE, NP, ts, batch_size(porder), &plan))) return st;
-    if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+    if ((st = get_geo<TG>(dHe, vHe, dV, vV, NE, ts, &geo))) return st;
 
     DiffuArgs<TG, TF, TO, TU> a;
     a.nelem = NE;
     a.npoin = NP;
     a.geo = geo;
-    a.gvol = gvol;
     a.rho = (const TF *)dr;
     a.u = (const TU *)du;
     a.Tem = (const TF *)dT;
@@ -1115,15 +1111,14 @@ int closure_elem_common(int porder, int nelem, int npoin, const int32_t *connec,
     if ((st = resolve(rho, sizeof(double) * NP, "rho", &dr, &vx))) return st;
     if ((st = resolve(u, sizeof(double) * 3 * NP, "u", &du, &vx))) return st;
     const PlanEntry *plan;
-    const double *geo, *gvol;
+    const double *geo;
     if ((st = get_plan(dc, vc, NE, NP, ts, batch_size(porder), &plan))) return st;
-    if ((st = get_geo<double>(dHe, vHe, dV, vV, NE, ts, &geo, &gvol))) return st;
+    if ((st = get_geo<double>(dHe, vHe, dV, vV, NE, ts, &geo))) return st;
     a = ClosureElemArgs{};
     a.nelem = NE;
     a.npoin = NP;
     a.conn = plan->conn;
     a.geo = geo;
-    a.gvol = gvol;
     a.rho = (const double *)dr;
     a.u = (const double *)du;
     a.tab = Tables{ts->dD, ts->dl2g};
