@@ -153,7 +153,7 @@ def rounded_to_fp32(prob):
     return q
 
 
-@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (6, 3), (7, 3)])
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (4, 5), (6, 3), (7, 3)])
 def test_mixed_precision_error(gpu, p, n):
     """FP32 storage of every input, FP64 arithmetic: equal to the FP64 oracle
     on the same FP32-rounded inputs (1e-12), and within the FP32 rounding of
@@ -183,7 +183,7 @@ def test_mixed_precision_error(gpu, p, n):
 TOL_FP32 = 5e-6
 
 
-@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (6, 3), (7, 3)])
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (4, 5), (6, 3), (7, 3)])
 def test_fp32_arithmetic_error(gpu, p, n):
     """FP32 storage and FP32 gathers, fluxes and derivative sweeps, FP64 node
     sums and assembly: the error against the FP64 oracle on the FP32-rounded
@@ -209,7 +209,7 @@ def test_fp32_arithmetic_error(gpu, p, n):
 TOL_SPLIT = 1e-6  # accuracy bar of the FP32 split against the FP64 oracle (relative L2)
 
 
-@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (6, 3), (7, 3)])
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (4, 5), (6, 3), (7, 3)])
 def test_f32split_error(gpu, p, n):
     """FP32 He, gpvol, mu_e, mu_sgs, q, rho, E, Tem, mu_fluid; FP64 u and pr;
     FP64 arithmetic: equal to the FP64 oracle on the same rounded inputs
@@ -271,7 +271,7 @@ def gpu_residuals_geo32(prob):
     return R + D
 
 
-@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (6, 3), (7, 3)])
+@pytest.mark.parametrize("p,n", [(3, 8), (4, 6), (4, 5), (6, 3), (7, 3)])
 def test_geo32_precision_error(gpu, p, n):
     """FP32 element arrays (He, gpvol, mu_e, mu_sgs), FP64 nodal fields and
     arithmetic: within 1e-6 of the FP64 oracle on the FP64 inputs, and equal to
